@@ -1,0 +1,290 @@
+"""Pins the CPU oracle (oracle/gridcpu.c) without trusting it.
+
+The reference ships no golden vectors, and its gridCPU cannot be compiled
+here without a stand-in for cuda.h (grid.cu:35), so every check below comes
+from something the oracle's own code does not share: closed forms (impulse
+response = the addressed sub-kernel, grid.cu:115-132), an independent numpy
+restatement by window slicing, integer coverage counts (the reference's
+DEBUG1 mode, grid.cu:127-129), linearity, brute force over tiny inputs and
+direct calls into glibc for the reference's input stream (grid.cu:444-452).
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def numpy_grid(uv, vals, N, gcf, S):
+    """Independent restatement: place each visibility's whole S x S window
+    by slicing, visibilities in ascending order (so per-pixel sums run in the
+    same order as the reference loop and the result is bit-identical)."""
+    h = S // 2
+    re = np.zeros((N, N))
+    im = np.zeros((N, N))
+    G = np.asarray(gcf, np.float64).reshape(64, S, S, 2)
+    for (u, v), (r1, i1) in zip(np.asarray(uv), np.asarray(vals)):
+        mx, my = math.floor(u), math.floor(v)
+        sx, sy = math.floor(8 * (u - mx)), math.floor(8 * (v - my))
+        g = G[8 * sy + sx]
+        # window rows y = my-h+1 .. my+h, cols x = mx-h+1 .. mx+h
+        y0, x0 = my - h + 1, mx - h + 1
+        ys = slice(max(y0, 0), min(y0 + S, N))
+        xs = slice(max(x0, 0), min(x0 + S, N))
+        if ys.start >= ys.stop or xs.start >= xs.stop:
+            continue
+        gr = g[ys.start - y0: ys.stop - y0, xs.start - x0: xs.stop - x0]
+        re[ys, xs] += r1 * gr[..., 0] - i1 * gr[..., 1]
+        im[ys, xs] += r1 * gr[..., 1] + gr[..., 0] * i1
+    return np.stack([re, im])
+
+
+@pytest.mark.parametrize("S", [8, 16, 32])
+def test_impulse_is_the_addressed_subkernel(S):
+    N = 4 * S
+    G = O.init_gcf(S).reshape(64, S, S, 2)  # asymmetric: catches transposes
+    h = S // 2
+    for sx, sy, mx, my in [(0, 0, 2 * S, 2 * S), (3, 5, 2 * S + 1, 2 * S - 3), (7, 1, S, 3 * S - 1)]:
+        uv = np.array([[mx + sx / 8.0, my + sy / 8.0]])
+        vals = np.array([[1.0, 0.0]])
+        out = O.grid_zz(uv, vals, N, G.reshape(-1, 2), S)
+        win = out[:, my - h + 1: my + h + 1, mx - h + 1: mx + h + 1]
+        k = 8 * sy + sx
+        np.testing.assert_array_equal(win[0], G[k, :, :, 0])
+        np.testing.assert_array_equal(win[1], G[k, :, :, 1])
+        # nothing outside the window
+        mask = np.ones((N, N), bool)
+        mask[my - h + 1: my + h + 1, mx - h + 1: mx + h + 1] = False
+        assert not out[:, mask].any()
+        # imaginary unit value rotates the kernel: (0+1i)*g = -gi + i gr
+        out_i = O.grid_zz(uv, np.array([[0.0, 1.0]]), N, G.reshape(-1, 2), S)
+        wi = out_i[:, my - h + 1: my + h + 1, mx - h + 1: mx + h + 1]
+        np.testing.assert_array_equal(wi[0], -G[k, :, :, 1])
+        np.testing.assert_array_equal(wi[1], G[k, :, :, 0])
+
+
+@pytest.mark.parametrize("S,N,npts", [(8, 48, 300), (16, 64, 200), (32, 96, 120)])
+def test_matches_independent_slicing_restatement_bitwise(S, N, npts):
+    # uv spans beyond the grid on every side, so edge clipping is exercised
+    uv, vals = O.gen_uniform(npts, -S / 2.0, N + S / 2.0, seed=11 + S)
+    gcf = O.config_gcf(S)
+    got = O.grid_zz(uv, vals, N, gcf, S)
+    ref = numpy_grid(uv, vals, N, gcf, S)
+    np.testing.assert_array_equal(got, ref)
+    gi = O.init_gcf(S)
+    np.testing.assert_array_equal(O.grid_zz(uv, vals, N, gi, S), numpy_grid(uv, vals, N, gi, S))
+
+
+def test_coverage_counts_debug1():
+    S, N, npts = 16, 80, 500
+    uv, vals = O.gen_uniform(npts, -10, N + 10, seed=5)
+    out = O.grid_zz(uv, vals, N, O.config_gcf(S), S, debug1=True)
+    # independent count: pixel (x,y) is covered by vis n iff
+    # floor(u)-S/2+1 <= x <= floor(u)+S/2 (and the same in y)
+    m = np.floor(uv).astype(np.int64)
+    xs = np.arange(N)
+    cx = (xs[None, :] >= m[:, 0:1] - S // 2 + 1) & (xs[None, :] <= m[:, 0:1] + S // 2)
+    cy = (xs[None, :] >= m[:, 1:2] - S // 2 + 1) & (xs[None, :] <= m[:, 1:2] + S // 2)
+    cnt = np.einsum("ny,nx->yx", cy.astype(np.int64), cx.astype(np.int64))
+    idx = np.einsum("ny,nx->yx", cy * np.arange(npts)[:, None], cx.astype(np.int64))
+    np.testing.assert_array_equal(out[0], cnt.astype(np.float64))
+    np.testing.assert_array_equal(out[1], idx.astype(np.float64))
+
+
+def test_linearity_and_pz_accumulation():
+    S, N = 16, 64
+    uv, v1 = O.gen_uniform(200, 0, N, seed=1)
+    _, v2 = O.gen_uniform(200, 0, N, seed=2)
+    gcf = O.config_gcf(S)
+    g1 = O.grid_zz(uv, v1, N, gcf, S)
+    g2 = O.grid_zz(uv, v2, N, gcf, S)
+    g12 = O.grid_zz(uv, 2.5 * v1 + v2, N, gcf, S)
+    assert O.rel_errors(g12, 2.5 * g1 + g2)[0] < 1e-13
+    # gridCPU_pz adds into what is there (grid.cu:150 does not zero)
+    acc = g1.copy()
+    O.grid_zz(uv, v2, N, gcf, S, out=acc)
+    assert O.rel_errors(acc, g1 + g2)[0] < 1e-14
+
+
+def test_row_bands_and_openmp_are_bit_identical():
+    S, N = 32, 256
+    uv, vals = O.gen_clustered(3000, N, S, mean_r=40.0, seed=9)
+    gcf = O.config_gcf(S)
+    ref = O.grid_zz(uv, vals, N, gcf, S)
+    got, used = O.grid_zz_omp(uv, vals, N, gcf, S, 4)
+    assert used >= 1
+    np.testing.assert_array_equal(got, ref)
+    part = O.grid_zz_rows(uv, vals, N, gcf, S, 0, 100)
+    O.grid_zz_rows(uv, vals, N, gcf, S, 100, N, out=part)
+    np.testing.assert_array_equal(part, ref)
+
+
+def test_floorf_emulation_K4():
+    # float-representable uv: floorf path == exact path
+    uv, _ = O.gen_uniform(1000, 0, 2048, seed=3)
+    assert O.floorf_divergent(uv) == 0
+    # a double just below an integer rounds up in float: floorf((float)u) = m+1,
+    # so 8*(u - (m+1)) < 0 and the reference's sub index becomes -1
+    u = 100.0 - 1e-9
+    bad = np.array([[u, 50.25]])
+    assert O.floorf_divergent(bad) == 1
+    # just below a sub-pixel boundary: the float rounding of 8*frac gives 8
+    u2 = 7.0 + (1.0 - 2.0 ** -30)
+    assert O.floorf_divergent(np.array([[3.5, u2]])) == 1
+    # the emulated gridder skips a visibility whose emulated k is outside [0,64)
+    S, N = 8, 32
+    g_exact = O.grid_zz(np.array([[16.0 - 1e-9, 16.5]]), np.array([[1.0, 0.0]]), N, O.config_gcf(S), S)
+    g_emul = O.grid_zz(np.array([[16.0 - 1e-9, 16.5]]), np.array([[1.0, 0.0]]), N, O.config_gcf(S), S,
+                       floorf_emul=True)
+    assert np.abs(g_exact).sum() > 0
+    # sub_y = 4 (exact), sub_x = -1 (emulated) -> k = 31 instead of 39: a
+    # different sub-kernel, shifted main pixel unchanged
+    assert not np.array_equal(g_exact, g_emul)
+
+
+def test_init_gcf_closed_form():
+    S = 16
+    g = O.init_gcf(S)
+    for sub_x, sub_y, x, y in [(0, 0, 0, 0), (3, 5, 7, 2), (7, 7, 15, 15), (1, 6, 9, 0)]:
+        tmp = math.sin(6.28 * x / S / 8) * math.exp(-(1.0 * x * x + 1.0 * y * y * sub_y) / S / S / 2)
+        i = S * S * (sub_x + sub_y * 8) + x + y * S
+        assert g[i, 0] == tmp * math.sin(1.0 * x * sub_x / (y + 1)) + 0.4
+        assert g[i, 1] == tmp * math.cos(1.0 * x * sub_x / (y + 1)) - 0.2
+
+
+@pytest.mark.parametrize("S", [8, 64])
+def test_config_gcf_properties(S):
+    g = O.config_gcf(S)
+    G = (g[:, 0] + 1j * g[:, 1]).reshape(64, S, S)
+    h = S // 2
+    # tap (r,c) = (h-1,h-1) of sub-kernel 0 sits on the visibility: value 1
+    assert G[0, h - 1, h - 1] == 1.0
+    # separable: every sub-kernel is rank one
+    for k in (0, 9, 63):
+        s = np.linalg.svd(G[k], compute_uv=False)
+        assert s[1] / s[0] < 1e-13
+    # sub-kernel (sx,sy) is the transpose of (sy,sx)
+    np.testing.assert_allclose(G[8 * 3 + 5], G[8 * 5 + 3].T, rtol=0, atol=1e-15)
+    # modulus is the Gaussian of the distance
+    sx, sy, r, c = 2, 6, 5, S - 3
+    dx, dy = (c - h + 1) - sx / 8, (r - h + 1) - sy / 8
+    assert abs(abs(G[8 * sy + sx, r, c]) - math.exp(-(dx * dx + dy * dy) / (2 * (S / 6) ** 2))) < 1e-15
+
+
+def test_reference_input_stream_is_glibc_rand():
+    libc = ctypes.CDLL("libc.so.6")
+    libc.rand.restype = ctypes.c_int
+    uv, vals = O.ref_inputs(50, 2048)
+    libc.srand(1541617)
+    RAND_MAX = 2147483647
+    f32 = np.float32
+    for n in range(50):
+        x = f32(f32(libc.rand()) / f32(RAND_MAX)) * f32(2048)
+        y = f32(f32(libc.rand()) / f32(RAND_MAX)) * f32(2048)
+        vr = f32(libc.rand()) / f32(RAND_MAX)
+        vi = f32(libc.rand()) / f32(RAND_MAX)
+        assert uv[n, 0] == float(x) and uv[n, 1] == float(y)
+        assert vals[n, 0] == float(vr) and vals[n, 1] == float(vi)
+
+
+def test_generators():
+    N, S = 1024, 64
+    uv, vals = O.gen_clustered(20000, N, S, mean_r=100.0, seed=4)
+    m = np.floor(uv)
+    assert (m - S // 2 + 1 >= 0).all() and (m + S // 2 <= N - 1).all()
+    assert (uv.astype(np.float32).astype(np.float64) == uv).all()
+    assert abs(vals.mean()) < 0.02 and abs(vals.std() - 1) < 0.02
+    r = np.hypot(uv[:, 0] - N / 2, uv[:, 1] - N / 2)
+    assert 90 < r.mean() < 110
+    uv2, _ = O.gen_clustered(20000, N, S, mean_r=100.0, seed=4)
+    np.testing.assert_array_equal(uv, uv2)
+    uvu, _ = O.gen_uniform(5000, 64, 1984, seed=7)
+    assert uvu.min() >= 64 and uvu.max() < 1984
+
+
+def test_vis2ints_and_set_bookmarks():
+    N, S = 256, 32
+    uv, _ = O.gen_uniform(3000, 0, N, seed=8)
+    ints = O.vis2ints(uv)
+    np.testing.assert_array_equal(ints[:, 0], np.floor(uv[:, 0]).astype(int) * 8 + np.floor(8 * (uv[:, 0] - np.floor(uv[:, 0]))).astype(int))
+    bs = S // 2
+    bg = (N + bs - 1) // bs
+    cell = (ints[:, 1] // 8 // bs) * bg + ints[:, 0] // 8 // bs
+    order = np.argsort(cell, kind="stable")
+    ints_s = ints[order]
+    bm = O.set_bookmarks(ints_s, bs, bg)
+    np.testing.assert_array_equal(bm, np.searchsorted(cell[order], np.arange(bg * bg + 1), side="left"))
+
+
+@pytest.mark.parametrize("N,S,H", [(128, 16, 16), (200, 32, 32), (256, 64, 32), (64, 8, 8)])
+def test_bin_sort(N, S, H):
+    uv, _ = O.gen_uniform(4000, -S, N + S, seed=N + S)
+    perm, offs = O.bin_sort(uv, N, S, H)
+    nb = O.n_bins(N, S, H)
+    # independent key: q = floor(u)+S/2 in [0, N+S-2]
+    m = np.floor(uv).astype(np.int64)
+    sub = np.floor(8 * (uv - m)).astype(np.int64)
+    q = m + S // 2
+    ok = (q >= 0).all(1) & (q <= N + S - 2).all(1)
+    key = (8 * sub[:, 1] + sub[:, 0]) * nb * nb + (q[:, 1] // H) * nb + q[:, 0] // H
+    idx = np.nonzero(ok)[0]
+    exp_perm = idx[np.argsort(key[idx], kind="stable")]
+    np.testing.assert_array_equal(perm, exp_perm)
+    np.testing.assert_array_equal(offs, np.searchsorted(np.sort(key[idx]), np.arange(64 * nb * nb + 1)))
+    # every dropped visibility really touches no pixel
+    for n in np.nonzero(~ok)[0][:50]:
+        assert np.abs(numpy_grid(uv[n:n + 1], np.array([[1.0, 1.0]]), N, O.config_gcf(S), S)).sum() == 0
+
+
+def test_window_pixel_brute_force():
+    S, N = 16, 100
+    for m in range(-5, N + 5):
+        for t in range(S):
+            p = O.window_pixel(m, t, S, N)
+            cands = [x for x in range(m - S // 2 + 1, m + S // 2 + 1) if x % S == t]
+            if m + S // 2 < t:
+                assert p == -1
+            else:
+                assert len(cands) == 1
+                assert p == (cands[0] if cands[0] < N else -1)
+
+
+def test_compute_gcf_phase_float_semantics():
+    uv, vals = O.gen_uniform(64, 0, 128, seed=12)
+    T, w = 0.7, 0.31
+    r2, i2 = O.compute_gcf_phase(uv, T, w)
+    f32 = np.float32
+    p1 = f32(2 * 3.1415926 * w)
+    p2 = f32(float(p1) * T)
+    m = np.floor(uv)
+    sub = np.floor(8 * (uv - m))
+    xs = (m[:, 0] - uv[:, 0] + sub[:, 0] / 8.0).astype(f32)
+    ys = (m[:, 1] - uv[:, 1] + sub[:, 1] / 8.0).astype(f32)
+    ph = p1 - p2 * np.sqrt(xs * xs + ys * ys)
+    np.testing.assert_allclose(r2, np.sin(ph.astype(np.float64)), atol=2e-7)
+    np.testing.assert_allclose(i2, np.cos(ph.astype(np.float64)), atol=2e-7)
+    S, N = 8, 128
+    g = O.grid_const_tap(uv, vals, N, S, r2, i2)
+    # independent: constant tap value = impulse with an all-ones table times value
+    tbl = np.zeros((64 * S * S, 2))
+    tbl[:, 0] = 1.0
+    v = (vals[:, 0] + 1j * vals[:, 1]) * (r2.astype(np.float64) + 1j * i2.astype(np.float64))
+    ref = numpy_grid(uv, np.stack([v.real, v.imag], 1), N, tbl, S)
+    assert O.rel_errors(g, ref)[0] < 1e-14
+
+
+def test_float_pairs_close_to_double():
+    S, N = 16, 96
+    uv, vals = O.gen_uniform(500, 0, N, seed=21)
+    gcf = O.config_gcf(S)
+    ref = O.grid_zz(uv, vals, N, gcf, S)
+    zc = O.grid_zc(uv, vals, N, gcf, S)
+    cc = O.grid_cc(uv, vals, N, gcf, S)
+    assert O.rel_errors(zc, ref)[0] < 1e-6
+    assert O.rel_errors(cc, ref)[0] < 1e-5
+    # zc sums float products in double: equal to a numpy restatement of that
+    G = gcf.astype(np.float32).reshape(64, S, S, 2)
+    assert zc.dtype == np.float64 and cc.dtype == np.float32
+    assert G.shape[0] == 64
