@@ -1,0 +1,247 @@
+#!/usr/bin/env python3
+"""Benchmark: Mvis/s of the gridder on an 8192^2 grid, support 64, 10M
+radially clustered visibilities (FP32-tiled vs FP64 plain scatter, with
+per-step p50/p99 latency), one JSON line on stdout.
+
+A step is one full gridding pass over the 10M-visibility batch resident in
+HBM: grid zeroing, bin sort (tiled paths) and the gridding kernels.  The
+timed region is bracketed by torch.cuda.synchronize() on both sides; each
+step is also timed with HIP events for the latency percentiles.
+
+Inputs are synthetic, drawn here with a counter-based SplitMix64 stream
+(seed 1541618): angle U[0, 2pi), radius Exp(mean 800 px) around the grid
+centre, redrawn until the 64 x 64 window lies on the grid, uv rounded to
+float32-representable values, values complex N(0,1) (Box-Muller).  The GCF
+is the separable Gaussian (sigma = S/6) x exp(0.05 i r^2) table.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ska-gpu-grid_amd"))
+
+METRIC = "Mvis/s at 8192² grid, support 64: FP32-tiled vs FP64 plain; p99 latency"
+PEAK_FP64_TF = 78.6   # MI355X FP64 vector/matrix, dense
+PEAK_FP32_TF = 157.3  # MI355X FP32 vector (packed), dense
+PEAK_HBM_GBS = 8000.0
+
+_G = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix_u01(seed, counter):
+    """Uniform [0,1) doubles from a counter-based SplitMix64 stream."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + _G * (counter.astype(np.uint64) + np.uint64(1))
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def gen_clustered(npts, N, S, mean_r=800.0, seed=1541618):
+    h = S // 2
+    uv = np.empty((npts, 2), np.float64)
+    todo = np.arange(npts)
+    rnd = 0
+    while todo.size:
+        c = (np.int64(rnd) * npts * 2 + np.arange(todo.size) * 2).astype(np.uint64)
+        th = 2 * math.pi * splitmix_u01(seed, c)
+        r = -mean_r * np.log1p(-splitmix_u01(seed, c + np.uint64(1)))
+        u = (N / 2 + r * np.cos(th)).astype(np.float32).astype(np.float64)
+        v = (N / 2 + r * np.sin(th)).astype(np.float32).astype(np.float64)
+        fu, fv = np.floor(u), np.floor(v)
+        ok = (fu - h + 1 >= 0) & (fu + h <= N - 1) & (fv - h + 1 >= 0) & (fv + h <= N - 1)
+        uv[todo[ok], 0] = u[ok]
+        uv[todo[ok], 1] = v[ok]
+        todo = todo[~ok]
+        rnd += 1
+    c = np.arange(npts, dtype=np.uint64) * np.uint64(2) + np.uint64(1 << 40)
+    u1 = 1.0 - splitmix_u01(seed, c)
+    u2 = splitmix_u01(seed, c + np.uint64(1))
+    rr = np.sqrt(-2.0 * np.log(u1))
+    vals = np.stack([rr * np.cos(2 * math.pi * u2), rr * np.sin(2 * math.pi * u2)], 1)
+    return uv, vals
+
+
+def config_gcf(S, beta=0.05):
+    sigma = S / 6.0
+    sy, sx, r, c = np.meshgrid(np.arange(8), np.arange(8), np.arange(S), np.arange(S), indexing="ij")
+    dx = (c - S // 2 + 1) - sx / 8.0
+    dy = (r - S // 2 + 1) - sy / 8.0
+    q = dx * dx + dy * dy
+    amp = np.exp(-q / (2 * sigma * sigma))
+    g = np.stack([amp * np.cos(beta * q), amp * np.sin(beta * q)], -1)
+    return g.reshape(-1, 2)
+
+
+def pct(xs, p):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = (len(xs) - 1) * p / 100.0
+    lo, hi = int(math.floor(k)), int(math.ceil(k))
+    return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def time_variant(torch, ctx, fn, warmup, steps):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ctx.profiling(True)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = [a.elapsed_time(b) for a, b in evs]
+    kern = ctx.kernel_ms()
+    ctx.profiling(False)
+    return wall, per, kern
+
+
+def cpu_baseline(uv, vals, gcf, N, S, nsample):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the CPU restatement of the reference gridCPU
+    t0 = time.perf_counter()
+    O.grid_zz(uv[:nsample], vals[:nsample], N, gcf, S)
+    dt = time.perf_counter() - t0
+    return {"value": nsample / dt / 1e6, "unit": "Mvis/s", "cores": 1, "kind": "port",
+            "sample": f"first {nsample} visibilities of the same 8192^2/S=64 workload through the "
+                      f"single-threaded gridCPU restatement (reference loop order), {dt:.1f} s"}
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return {}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--npts", type=int, default=10_000_000)
+    ap.add_argument("--N", type=int, default=8192)
+    ap.add_argument("--S", type=int, default=64)
+    ap.add_argument("--cpu-sample", type=int, default=40000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--variants", default="fp64_plain,fp64_tiled,fp32_tiled,bf16_tiled")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported "
+                                   "(multi-GPU sharding is described, not built)" % args.gpus}))
+        sys.exit(2)
+
+    import torch
+    import migrid
+
+    N, S, npts = args.N, args.S, args.npts
+    t0 = time.perf_counter()
+    uv, vals = gen_clustered(npts, N, S)
+    gcf = config_gcf(S)
+    gen_s = time.perf_counter() - t0
+    print(f"# generated {npts} visibilities in {gen_s:.1f} s", file=sys.stderr, flush=True)
+
+    d_uv = torch.from_numpy(uv).cuda()
+    d_vals = torch.from_numpy(vals).cuda()
+    d_gcf = torch.from_numpy(gcf).cuda()
+    grid = torch.empty((N, N, 2), dtype=torch.float64, device="cuda")
+    ctx = migrid.Context()
+
+    variants = {
+        "fp64_plain": dict(strategy="scatter"),
+        "fp64_tiled": dict(strategy="tiled", arith=migrid.ARITH_NATIVE),
+        "fp32_tiled": dict(strategy="tiled", arith=migrid.ARITH_F32),
+        "bf16_tiled": dict(strategy="tiled", arith=migrid.ARITH_F32_BF16GCF),
+    }
+    results = {}
+    ref_grid = None
+    for name in args.variants.split(","):
+        kw = variants[name]
+
+        def step(kw=kw):
+            ctx.grid(grid, d_uv, d_vals, d_gcf, N, S, **kw)
+        try:
+            step()
+        except migrid.MigridError as e:
+            print(f"# {name}: not available ({e})", file=sys.stderr, flush=True)
+            continue
+        torch.cuda.synchronize()
+        print(f"# {name}: timing {args.steps} steps", file=sys.stderr, flush=True)
+        wall, per, kern = time_variant(torch, ctx, step, args.warmup, args.steps)
+        g = grid.clone()
+        if ref_grid is None and name == "fp64_plain":
+            ref_grid = g
+        err = None
+        if ref_grid is not None and name != "fp64_plain":
+            d = torch.linalg.vector_norm(g - ref_grid, dim=-1)
+            m = torch.linalg.vector_norm(ref_grid, dim=-1)
+            err = {"rel_max_vs_fp64_plain": float(d.max() / m.max()),
+                   "rel_rms_vs_fp64_plain": float(torch.sqrt((d * d).sum() / (m * m).sum()))}
+        ms = wall * 1e3 / args.steps
+        results[name] = {"ms_per_step": ms, "mvis_per_s": npts / (ms * 1e-3) / 1e6,
+                         "p50_ms": pct(per, 50), "p99_ms": pct(per, 99),
+                         "kernel_ms_avg": float(np.mean(kern)) if kern else None,
+                         "kernel_launches": len(kern), "err": err}
+        print(f"# {name}: {results[name]}", file=sys.stderr, flush=True)
+        del g
+    if not results:
+        print(json.dumps({"error": "no variant ran"}))
+        sys.exit(1)
+
+    head = "fp32_tiled" if "fp32_tiled" in results else next(iter(results))
+    R = results[head]
+    flops = 8.0 * S * S * npts
+    kms = R["kernel_ms_avg"]
+    fp32 = head in ("fp32_tiled", "bf16_tiled")
+    peak = PEAK_FP32_TF if fp32 else PEAK_FP64_TF
+    achieved = flops / (kms * 1e-3) / 1e12 if kms else None
+    traffic = load_traffic().get(head)
+    out = {
+        "metric": METRIC,
+        "value": R["mvis_per_s"],
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": R["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if fp32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{N}x{N} complex128 grid, support {S}, oversampling 8, "
+                               f"{npts} radially clustered visibilities (Exp radius mean 800 px)",
+                   "variant": head, "grid": N, "support": S, "npts": npts},
+        "p50_ms": R["p50_ms"],
+        "p99_ms": R["p99_ms"],
+        "variants": results,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak if achieved else None, "traffic": traffic,
+                     "kernel": "tiled owner kernel" if head != "fp64_plain" else "plain scatter kernel",
+                     "flops_per_launch": flops},
+    }
+    if "fp64_plain" in results and head != "fp64_plain":
+        out["speedup_vs_fp64_plain"] = R["mvis_per_s"] / results["fp64_plain"]["mvis_per_s"]
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(uv, vals, gcf, N, S, args.cpu_sample)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,193 @@
+/*
+ * migrid.h -- C ABI of the MI355X (gfx950) convolutional gridder.
+ *
+ * Drop-in replacement for the reference's gridding boundary:
+ *   template <class CmplxOutType, class CmplxType>
+ *   void gridGPU(CmplxOutType* out, CmplxType* in, CmplxType* in_vals,
+ *                size_t npts, size_t img_dim, CmplxType* gcf, size_t gcf_dim);
+ *        (grid_gpu.cuh:30-32, body grid_gpu.cu:675-819, instantiated for
+ *         <double2,double2>, <double2,float2>, <float2,float2> at :820-825)
+ *   void gridCPU   (PRECISION2* out, ...)      (grid.cu:87-139)
+ *   void gridCPU_pz(OUTPRECISION2* out, ...)   (grid.cu:140-193)
+ *
+ * Plain C types only: complex values are {x, y} pairs laid out like
+ * float2/double2.  Every entry point returns 0 on success or a negative
+ * MIGRID_E* code; migrid_last_error() gives the message (per thread).
+ * Oversampling is fixed at 8 (64 sub-kernels), one polarisation, as in the
+ * reference (Defines.h:60,77).  img_dim and gcf_dim are run-time values.
+ */
+#ifndef MIGRID_H
+#define MIGRID_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double x, y; } migrid_c128; /* layout of double2 */
+typedef struct { float x, y; } migrid_c64;   /* layout of float2 */
+
+/* ---- status codes ------------------------------------------------------- */
+#define MIGRID_OK 0
+#define MIGRID_EINVAL (-1)  /* bad argument or shape */
+#define MIGRID_ENOMEM (-2)  /* device or host allocation failed */
+#define MIGRID_EDEVICE (-3) /* HIP runtime error (message has the HIP string) */
+#define MIGRID_ENOTSUP (-4) /* combination not supported */
+#define MIGRID_EBUSY (-5)   /* admission refused: not enough free HBM */
+
+/* ---- options ------------------------------------------------------------ */
+/* Strategy.  SCATTER is the reference default (grid_gpu.cu:778-786): one
+ * hardware atomic per tap.  GATHER and WINDOW are the reference's other two
+ * (:734-776), rebuilt as per-tile owner kernels.  TILED is the bin-sorted
+ * LDS-tiled gridder with atomics only where a tile is flushed. */
+#define MIGRID_SCATTER 0
+#define MIGRID_GATHER 1
+#define MIGRID_WINDOW 2
+#define MIGRID_TILED 3
+
+/* Output layout.  DEFAULT follows the reference per strategy: interleaved
+ * {re,im} for scatter/gather (grid_gpu.cu:162-163,500), planar (real plane
+ * then imaginary plane) for the moving window (:524-525,585-586) and for the
+ * gridCPU entry points (grid.cu:101-102,153-154).  TILED defaults to
+ * interleaved. */
+#define MIGRID_LAYOUT_DEFAULT 0
+#define MIGRID_LAYOUT_INTERLEAVED 1
+#define MIGRID_LAYOUT_PLANAR 2
+
+/* GCF source.  TABLE reads gcf[64*S*S].  COMPAT is the reference's
+ * COMPUTE_GCF (grid_gpu.cu:103-108,137-147): T = gcf[S(S-1)/2-1].x,
+ * w = .y, every tap of a visibility uses (sinf, cosf)(phase).  ANALYTIC
+ * evaluates exp(-(dx^2+dy^2)/(2 sigma^2)) * exp(i beta (dx^2+dy^2)) at the
+ * table's quantised offsets on the device (no table is read). */
+#define MIGRID_GCF_TABLE 0
+#define MIGRID_GCF_COMPAT 1
+#define MIGRID_GCF_ANALYTIC 2
+
+/* Tile arithmetic (TILED strategy).  NATIVE: products in the input type,
+ * sums in the output type.  F32: vis x GCF products and in-tile sums in
+ * FP32.  F32_BF16GCF: the same with the GCF staged as bfloat16. */
+#define MIGRID_ARITH_NATIVE 0
+#define MIGRID_ARITH_F32 1
+#define MIGRID_ARITH_F32_BF16GCF 2
+
+typedef struct migrid_opts {
+    int32_t strategy;       /* MIGRID_SCATTER .. MIGRID_TILED */
+    int32_t layout;         /* MIGRID_LAYOUT_* */
+    int32_t gcf_mode;       /* MIGRID_GCF_* */
+    int32_t arith;          /* MIGRID_ARITH_* (TILED only) */
+    int32_t accumulate;     /* 0: fresh grid (zeroed first), 1: add into it */
+    int32_t debug_coverage; /* 1: every tap adds (1, n) -- the reference's
+                               DEBUG1 coverage map (grid.cu:127-129) */
+    int32_t tile;           /* TILED/GATHER/WINDOW tile edge in pixels, 0 = auto */
+    int32_t vis_per_wave;   /* TILED work-item size, 0 = auto */
+    double gcf_sigma;       /* ANALYTIC: Gaussian sigma in pixels (0: S/6) */
+    double gcf_beta;        /* ANALYTIC: quadratic phase coefficient */
+    int32_t reserved[8];
+} migrid_opts;
+
+/* Fill *o with the defaults (scatter, default layout, table GCF, fresh grid). */
+void migrid_opts_default(migrid_opts* o);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* migrid_last_error(void);
+const char* migrid_version(void);
+
+/* ---- drop-in host entry points ------------------------------------------ */
+/* gridGPU<double2,double2> / <double2,float2> / <float2,float2>
+ * (grid_gpu.cu:675-819).  `out` is the PADDED host buffer of
+ * (img_dim^2 + 2*img_dim*gcf_dim + 2*gcf_dim) complex elements; the grid base
+ * is out + img_dim*gcf_dim + gcf_dim (grid_gpu.cu:711,732).  The whole padded
+ * buffer is zeroed on the device, gridded and copied back (unless
+ * opts->accumulate, which grids into the values already in `out`).
+ * in[n] = {u, v} in pixels, in_vals[n] paired by index, gcf[64*gcf_dim^2].
+ * opts may be NULL (reference semantics: scatter, interleaved). */
+int migrid_gridGPU_zz(migrid_c128* out, const migrid_c128* in, const migrid_c128* in_vals,
+                      size_t npts, size_t img_dim, const migrid_c128* gcf, size_t gcf_dim,
+                      const migrid_opts* opts);
+int migrid_gridGPU_zc(migrid_c128* out, const migrid_c64* in, const migrid_c64* in_vals,
+                      size_t npts, size_t img_dim, const migrid_c64* gcf, size_t gcf_dim,
+                      const migrid_opts* opts);
+int migrid_gridGPU_cc(migrid_c64* out, const migrid_c64* in, const migrid_c64* in_vals,
+                      size_t npts, size_t img_dim, const migrid_c64* gcf, size_t gcf_dim,
+                      const migrid_opts* opts);
+
+/* gridCPU (grid.cu:87-139): `out` is the UNPADDED grid base; the result is
+ * PLANAR (real plane of img_dim^2 doubles, then the imaginary plane), zeroed
+ * first.  Computed on the GPU.  gridCPU_pz (grid.cu:140-193) is the same but
+ * adds into the planes already in `out`.  _zz: double everything; _zc:
+ * float inputs, double planes; _cc: float inputs, float planes. */
+int migrid_gridCPU_zz(migrid_c128* out, const migrid_c128* in, const migrid_c128* in_vals,
+                      size_t npts, size_t img_dim, const migrid_c128* gcf, size_t gcf_dim);
+int migrid_gridCPU_pz_zz(migrid_c128* out, const migrid_c128* in, const migrid_c128* in_vals,
+                         size_t npts, size_t img_dim, const migrid_c128* gcf, size_t gcf_dim);
+int migrid_gridCPU_pz_zc(migrid_c128* out, const migrid_c64* in, const migrid_c64* in_vals,
+                         size_t npts, size_t img_dim, const migrid_c64* gcf, size_t gcf_dim);
+int migrid_gridCPU_pz_cc(migrid_c64* out, const migrid_c64* in, const migrid_c64* in_vals,
+                         size_t npts, size_t img_dim, const migrid_c64* gcf, size_t gcf_dim);
+
+/* Size in complex elements of the padded buffer gridGPU expects. */
+size_t migrid_padded_elems(size_t img_dim, size_t gcf_dim);
+
+/* ---- device-resident entry points --------------------------------------- */
+/* Element types of a device job. */
+#define MIGRID_F64 0
+#define MIGRID_F32 1
+
+/* One gridding job on device memory.  All pointers are device pointers.
+ * grid is the UNPADDED base: N*N complex interleaved, or two planes of N*N
+ * (real then imaginary) when the layout is planar.  grid_lo is the
+ * compensation grid of the FP32 two-sum flush (same shape as grid), or NULL. */
+typedef struct migrid_job {
+    int32_t in_type;  /* MIGRID_F64 / MIGRID_F32: uv, vals and gcf */
+    int32_t out_type; /* MIGRID_F64 / MIGRID_F32: grid */
+    const void* uv;   /* npts x {u,v} */
+    const void* vals; /* npts x {re,im} */
+    const void* gcf;  /* 64*S*S x {re,im} (may be NULL for ANALYTIC) */
+    void* grid;
+    void* grid_lo;
+    int64_t npts;
+    int64_t img_dim;
+    int64_t gcf_dim;
+} migrid_job;
+
+/* A context owns device scratch (bin-sort buffers, work queues), reused
+ * across calls.  One context is used by one host thread at a time; calls on
+ * one context are ordered on the stream passed to them. */
+typedef struct migrid_ctx migrid_ctx;
+
+int migrid_ctx_create(migrid_ctx** ctx, int device);
+int migrid_ctx_destroy(migrid_ctx* ctx);
+
+/* Grid one job on `stream` (a hipStream_t, or NULL for the null stream).
+ * Asynchronous with respect to the host. */
+int migrid_ctx_grid(migrid_ctx* ctx, const migrid_job* job, const migrid_opts* opts,
+                    void* stream);
+
+/* Bin sort only (device pointers): stable order of visibility indices by
+ * key (k, bin_y, bin_x) with k = 8*sub_y + sub_x, bins of `bin` pixels over
+ * q = floor(u) + S/2 in [0, N+S-2]; perm[0..count) and CSR offsets
+ * [0..64*nb*nb] with nb = (N+S-2)/bin + 1.  *count receives the number of
+ * binned visibilities (those touching the grid).  Synchronous. */
+int migrid_ctx_bin_sort(migrid_ctx* ctx, int32_t in_type, const void* uv, int64_t npts,
+                        int64_t img_dim, int64_t gcf_dim, int32_t bin, int32_t* perm,
+                        int32_t* offsets, int64_t* count, void* stream);
+
+/* Kernel timing: when enabled, every call records HIP events (on the call's
+ * stream) around the main gridding kernel.  migrid_ctx_profile_read
+ * synchronises on them and copies up to `cap` per-launch durations (ms) into
+ * ms_out (may be NULL); it returns the number of launches recorded since the
+ * last reset, or a negative status. */
+int migrid_ctx_set_profiling(migrid_ctx* ctx, int enable);
+int migrid_ctx_profile_read(migrid_ctx* ctx, double* ms_out, int cap);
+int migrid_ctx_profile_reset(migrid_ctx* ctx);
+
+/* Number of bins per axis for migrid_ctx_bin_sort. */
+int64_t migrid_bins_per_axis(int64_t img_dim, int64_t gcf_dim, int32_t bin);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* MIGRID_H */

@@ -1,0 +1,298 @@
+"""Python binding over the libmigrid C ABI (include/migrid.h).
+
+Replaces the reference's Python-2 extension GPUGrid.convgrid
+(GPUGrid.cpp:124-164) with a ctypes binding: numpy arrays in, the padded or
+unpadded grid out, errors raised instead of printed (GPUGrid.cpp:128-141),
+and a correctly sized output (the reference allocates 16*npts bytes for an
+img_size^2 grid, GPUGrid.cpp:148,159-160).
+
+Device-resident calls take torch tensors (device memory and streams are
+torch's; the computation is the HIP library's).  There is no CPU fallback:
+if libmigrid.so is missing or cannot load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmigrid.so")
+
+SCATTER, GATHER, WINDOW, TILED = 0, 1, 2, 3
+LAYOUT_DEFAULT, LAYOUT_INTERLEAVED, LAYOUT_PLANAR = 0, 1, 2
+GCF_TABLE, GCF_COMPAT, GCF_ANALYTIC = 0, 1, 2
+ARITH_NATIVE, ARITH_F32, ARITH_F32_BF16GCF = 0, 1, 2
+F64, F32 = 0, 1
+
+STRATEGIES = {"scatter": SCATTER, "gather": GATHER, "window": WINDOW, "tiled": TILED}
+
+_ERRNAMES = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ENOTSUP", -5: "EBUSY"}
+
+
+class MigridError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"migrid {_ERRNAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [
+        ("strategy", ctypes.c_int32),
+        ("layout", ctypes.c_int32),
+        ("gcf_mode", ctypes.c_int32),
+        ("arith", ctypes.c_int32),
+        ("accumulate", ctypes.c_int32),
+        ("debug_coverage", ctypes.c_int32),
+        ("tile", ctypes.c_int32),
+        ("vis_per_wave", ctypes.c_int32),
+        ("gcf_sigma", ctypes.c_double),
+        ("gcf_beta", ctypes.c_double),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [
+        ("in_type", ctypes.c_int32),
+        ("out_type", ctypes.c_int32),
+        ("uv", ctypes.c_void_p),
+        ("vals", ctypes.c_void_p),
+        ("gcf", ctypes.c_void_p),
+        ("grid", ctypes.c_void_p),
+        ("grid_lo", ctypes.c_void_p),
+        ("npts", ctypes.c_int64),
+        ("img_dim", ctypes.c_int64),
+        ("gcf_dim", ctypes.c_int64),
+    ]
+
+
+# Every symbol include/migrid.h declares (checked by the CPU test suite).
+EXPORTS = [
+    "migrid_opts_default", "migrid_last_error", "migrid_version",
+    "migrid_gridGPU_zz", "migrid_gridGPU_zc", "migrid_gridGPU_cc",
+    "migrid_gridCPU_zz", "migrid_gridCPU_pz_zz", "migrid_gridCPU_pz_zc", "migrid_gridCPU_pz_cc",
+    "migrid_padded_elems",
+    "migrid_ctx_create", "migrid_ctx_destroy", "migrid_ctx_grid",
+    "migrid_ctx_bin_sort", "migrid_bins_per_axis",
+    "migrid_ctx_set_profiling", "migrid_ctx_profile_read", "migrid_ctx_profile_reset",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libmigrid.so (raises if it is missing: no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MigridError(-3, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz, i64, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int64, ctypes.c_int32
+    L.migrid_opts_default.argtypes = [ctypes.POINTER(Opts)]
+    L.migrid_opts_default.restype = None
+    L.migrid_last_error.restype = ctypes.c_char_p
+    L.migrid_version.restype = ctypes.c_char_p
+    for name in ("migrid_gridGPU_zz", "migrid_gridGPU_zc", "migrid_gridGPU_cc"):
+        f = getattr(L, name)
+        f.argtypes = [vp, vp, vp, sz, sz, vp, sz, ctypes.POINTER(Opts)]
+        f.restype = ctypes.c_int
+    for name in ("migrid_gridCPU_zz", "migrid_gridCPU_pz_zz", "migrid_gridCPU_pz_zc",
+                 "migrid_gridCPU_pz_cc"):
+        f = getattr(L, name)
+        f.argtypes = [vp, vp, vp, sz, sz, vp, sz]
+        f.restype = ctypes.c_int
+    L.migrid_padded_elems.argtypes = [sz, sz]
+    L.migrid_padded_elems.restype = sz
+    L.migrid_ctx_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+    L.migrid_ctx_destroy.argtypes = [vp]
+    L.migrid_ctx_grid.argtypes = [vp, ctypes.POINTER(Job), ctypes.POINTER(Opts), vp]
+    L.migrid_ctx_bin_sort.argtypes = [vp, i32, vp, i64, i64, i64, i32, vp, vp,
+                                      ctypes.POINTER(i64), vp]
+    L.migrid_ctx_set_profiling.argtypes = [vp, ctypes.c_int]
+    L.migrid_ctx_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    L.migrid_ctx_profile_reset.argtypes = [vp]
+    L.migrid_bins_per_axis.argtypes = [i64, i64, i32]
+    L.migrid_bins_per_axis.restype = i64
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise MigridError(rc, lib().migrid_last_error().decode())
+
+
+def version():
+    return lib().migrid_version().decode()
+
+
+def make_opts(strategy="scatter", layout=LAYOUT_DEFAULT, gcf_mode=GCF_TABLE, arith=ARITH_NATIVE,
+              accumulate=False, debug_coverage=False, tile=0, vis_per_wave=0, sigma=0.0,
+              beta=0.05):
+    o = Opts()
+    lib().migrid_opts_default(ctypes.byref(o))
+    o.strategy = STRATEGIES[strategy] if isinstance(strategy, str) else int(strategy)
+    o.layout = layout
+    o.gcf_mode = gcf_mode
+    o.arith = arith
+    o.accumulate = int(bool(accumulate))
+    o.debug_coverage = int(bool(debug_coverage))
+    o.tile = tile
+    o.vis_per_wave = vis_per_wave
+    o.gcf_sigma = sigma
+    o.gcf_beta = beta
+    return o
+
+
+def padded_elems(N, S):
+    return int(lib().migrid_padded_elems(N, S))
+
+
+def grid_base(N, S):
+    """Offset (complex elements) of the grid base inside the padded buffer."""
+    return N * S + S
+
+
+# ---- host (drop-in) calls -----------------------------------------------------
+
+def gridGPU(uv, vals, gcf, N, S, pair="zz", out=None, **kw):
+    """gridGPU<TO,T> (grid_gpu.cu:675-819) on numpy arrays.
+
+    uv, vals: (n,2); gcf: (64*S*S,2); pair in {"zz","zc","cc"} picks the
+    reference instantiation <double2,double2>, <double2,float2>,
+    <float2,float2>.  Returns the padded buffer as a (P,2) array (P =
+    padded_elems(N,S)); use unpad() for the grid."""
+    ti = np.float64 if pair == "zz" else np.float32
+    to = np.float32 if pair == "cc" else np.float64
+    uv = np.ascontiguousarray(uv, ti).reshape(-1, 2)
+    vals = np.ascontiguousarray(vals, ti).reshape(-1, 2)
+    gcf_p = None
+    if gcf is not None:
+        gcf = np.ascontiguousarray(gcf, ti).reshape(-1, 2)
+        if gcf.shape[0] != 64 * S * S:
+            raise MigridError(-1, f"gcf has {gcf.shape[0]} elements, expected 64*S*S={64 * S * S}")
+        gcf_p = gcf.ctypes.data
+    if uv.shape != vals.shape:
+        raise MigridError(-1, "uv and vals differ in length")
+    P = padded_elems(N, S)
+    if out is None:
+        out = np.zeros((P, 2), to)
+    elif out.shape != (P, 2) or out.dtype != to or not out.flags.c_contiguous:
+        raise MigridError(-1, f"out must be a contiguous ({P},2) {np.dtype(to).name} array")
+    o = make_opts(**kw)
+    f = getattr(lib(), f"migrid_gridGPU_{pair}")
+    check(f(out.ctypes.data, uv.ctypes.data, vals.ctypes.data, uv.shape[0], N, gcf_p, S,
+            ctypes.byref(o)))
+    return out
+
+
+def gridCPU(uv, vals, gcf, N, S, pair="zz", out=None):
+    """gridCPU / gridCPU_pz (grid.cu:87-193) computed on the GPU.  Returns
+    the planar grid (2,N,N).  With out=None the grid is fresh (gridCPU);
+    with out given the call adds into it (gridCPU_pz)."""
+    ti = np.float64 if pair == "zz" else np.float32
+    to = np.float32 if pair == "cc" else np.float64
+    uv = np.ascontiguousarray(uv, ti).reshape(-1, 2)
+    vals = np.ascontiguousarray(vals, ti).reshape(-1, 2)
+    gcf = np.ascontiguousarray(gcf, ti).reshape(-1, 2)
+    if out is None:
+        out = np.zeros((2, N, N), to)
+        name = "migrid_gridCPU_zz" if pair == "zz" else f"migrid_gridCPU_pz_{pair}"
+    else:
+        if out.shape != (2, N, N) or out.dtype != to or not out.flags.c_contiguous:
+            raise MigridError(-1, "out must be a contiguous (2,N,N) array of the output type")
+        name = f"migrid_gridCPU_pz_{pair}"
+    check(getattr(lib(), name)(out.ctypes.data, uv.ctypes.data, vals.ctypes.data, uv.shape[0], N,
+                               gcf.ctypes.data, S))
+    return out
+
+
+def unpad(padded, N, S, planar=False):
+    """Grid view of a padded buffer: complex (N,N) for interleaved, or the
+    planar (2,N,N) real/imaginary planes."""
+    base = grid_base(N, S)
+    flat = padded.reshape(-1)
+    if planar:
+        return flat[2 * base: 2 * base + 2 * N * N].reshape(2, N, N)
+    g = padded[base: base + N * N]
+    return (g[:, 0] + 1j * g[:, 1]).reshape(N, N)
+
+
+# ---- device-resident calls (torch tensors) ------------------------------------
+
+class Context:
+    """A migrid_ctx: device scratch reused across calls."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().migrid_ctx_create(ctypes.byref(h), device))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().migrid_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def grid(self, grid, uv, vals, gcf, N, S, grid_lo=None, stream=None, **kw):
+        """Grid on device tensors.  grid: unpadded (N*N*2,) or (N,N,2) /
+        (2,N,N) tensor of float64/float32; uv, vals: (n,2); gcf: (64*S*S,2)
+        or None (ANALYTIC).  Asynchronous on `stream` (default: torch's
+        current stream)."""
+        import torch
+        j = Job()
+        j.in_type = F64 if uv.dtype == torch.float64 else F32
+        j.out_type = F64 if grid.dtype == torch.float64 else F32
+        for t in (uv, vals, grid) + ((gcf,) if gcf is not None else ()):
+            if not t.is_contiguous() or not t.is_cuda:
+                raise MigridError(-1, "device tensors must be contiguous and on the GPU")
+        j.uv = uv.data_ptr()
+        j.vals = vals.data_ptr()
+        j.gcf = gcf.data_ptr() if gcf is not None else None
+        j.grid = grid.data_ptr()
+        j.grid_lo = grid_lo.data_ptr() if grid_lo is not None else None
+        j.npts = uv.shape[0]
+        j.img_dim = N
+        j.gcf_dim = S
+        o = make_opts(**kw)
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        check(lib().migrid_ctx_grid(self.h, ctypes.byref(j), ctypes.byref(o),
+                                    ctypes.c_void_p(stream.cuda_stream)))
+
+    def profiling(self, on=True):
+        """Record HIP events around the main gridding kernel of each call."""
+        check(lib().migrid_ctx_set_profiling(self.h, int(bool(on))))
+        check(lib().migrid_ctx_profile_reset(self.h))
+
+    def kernel_ms(self, cap=1 << 16):
+        """Per-launch durations (ms) of the main kernel since profiling()."""
+        buf = (ctypes.c_double * cap)()
+        n = lib().migrid_ctx_profile_read(self.h, buf, cap)
+        if n < 0:
+            check(n)
+        return [buf[i] for i in range(min(n, cap))]
+
+    def bin_sort(self, uv, N, S, bin_size, stream=None):
+        """Device bin sort; returns (perm, offsets) torch int32 tensors."""
+        import torch
+        nb = int(lib().migrid_bins_per_axis(N, S, bin_size))
+        n = uv.shape[0]
+        perm = torch.empty(max(n, 1), dtype=torch.int32, device=uv.device)
+        offs = torch.empty(64 * nb * nb + 1, dtype=torch.int32, device=uv.device)
+        cnt = ctypes.c_int64()
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        check(lib().migrid_ctx_bin_sort(self.h, F64 if uv.dtype == torch.float64 else F32,
+                                        uv.data_ptr(), n, N, S, bin_size, perm.data_ptr(),
+                                        offs.data_ptr(), ctypes.byref(cnt),
+                                        ctypes.c_void_p(stream.cuda_stream)))
+        return perm[: cnt.value], offs

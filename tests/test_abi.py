@@ -1,0 +1,69 @@
+"""CPU checks of the C ABI: the library loads, exports every symbol the
+header declares, and rejects bad arguments before touching a device."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import migrid
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^\s*(?:[A-Za-z_][\w \*]*?)\b(migrid_\w+)\s*\(", txt, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = migrid.lib()
+    decl = declared_symbols()
+    assert "migrid_gridGPU_zz" in decl and len(decl) >= 15
+    for name in sorted(decl):
+        assert hasattr(L, name), name
+    assert set(migrid.EXPORTS) <= decl
+
+
+def test_version_and_defaults():
+    assert "gfx950" in migrid.version()
+    o = migrid.make_opts()
+    assert o.strategy == migrid.SCATTER and o.layout == 0 and o.accumulate == 0
+    assert ctypes.sizeof(migrid.Opts) == 4 * 8 + 8 * 2 + 4 * 8
+    assert migrid.padded_elems(2048, 32) == 2048 * 2048 + 2 * 2048 * 32 + 2 * 32
+
+
+@pytest.mark.parametrize("N,S,npts", [(64, 7, 10), (64, 0, 10), (64, 130, 10), (0, 8, 10), (64, 8, -1)])
+def test_bad_shapes_rejected_before_any_device_call(N, S, npts):
+    L = migrid.lib()
+    buf = np.zeros(16, np.float64)
+    rc = L.migrid_gridGPU_zz(buf.ctypes.data, buf.ctypes.data, buf.ctypes.data,
+                             ctypes.c_size_t(npts & 0xFFFFFFFFFFFFFFFF), N, buf.ctypes.data, S, None)
+    assert rc == -1
+    assert L.migrid_last_error().decode()
+
+
+def test_null_pointers_rejected():
+    L = migrid.lib()
+    buf = np.zeros(16, np.float64)
+    assert L.migrid_gridGPU_zz(None, buf.ctypes.data, buf.ctypes.data, 1, 64, buf.ctypes.data, 8, None) == -1
+    assert "out" in L.migrid_last_error().decode()
+    assert L.migrid_gridGPU_cc(buf.ctypes.data, None, None, 3, 64, buf.ctypes.data, 8, None) == -1
+    assert L.migrid_gridGPU_zc(buf.ctypes.data, buf.ctypes.data, buf.ctypes.data, 1, 64, None, 8, None) == -1
+    assert L.migrid_ctx_grid(None, None, None, None) == -1
+
+
+def test_gcf_size_checked_by_binding():
+    with pytest.raises(migrid.MigridError):
+        migrid.gridGPU(np.zeros((1, 2)), np.zeros((1, 2)), np.zeros((10, 2)), 64, 8)
+
+
+def test_bins_per_axis():
+    L = migrid.lib()
+    assert L.migrid_bins_per_axis(8192, 64, 64) == (8192 + 62) // 64 + 1
+    assert L.migrid_bins_per_axis(100, 8, 0) == 0
