@@ -434,7 +434,10 @@ int64_t oracle_window_pixel(int64_t m, int64_t t, int64_t S, int64_t N)
  * phase = p1 - p2*sqrt(dx^2+dy^2) in float with the scatter kernel's
  * dx = (float)(main_x - u + sub_x/8); the tap value is (sin, cos)(phase),
  * the same for every tap of the visibility. */
-void oracle_compute_gcf_phase(const double* uv, int64_t npts, double T, double w,
+/* form 0: the scatter kernel's dx (float uv); form 1: the gather and
+ * moving-window kernels' dx, computed from the int2 fixed point
+ * inn = main*8+sub in place of u (grid_gpu.cu:456-459, :600-603). */
+void oracle_compute_gcf_phase(const double* uv, int64_t npts, double T, double w, int form,
                               float* r2_out, float* i2_out)
 {
     float p1 = (float)(2 * 3.1415926 * w);
@@ -443,8 +446,13 @@ void oracle_compute_gcf_phase(const double* uv, int64_t npts, double T, double w
         int mx, sx, my, sy;
         index_exact(uv[2 * n], &mx, &sx);
         index_exact(uv[2 * n + 1], &my, &sy);
-        float xs = (float)(mx - uv[2 * n] + sx * 1.0 / 8.0);
-        float ys = (float)(my - uv[2 * n + 1] + sy * 1.0 / 8.0);
+        double ux = uv[2 * n], uy = uv[2 * n + 1];
+        if (form == 1) {
+            ux = (double)(mx * OVERSAMPLE + sx);
+            uy = (double)(my * OVERSAMPLE + sy);
+        }
+        float xs = (float)(mx - ux + sx * 1.0 / 8.0);
+        float ys = (float)(my - uy + sy * 1.0 / 8.0);
         xs *= xs;
         ys *= ys;
         float ph = p1 - p2 * sqrtf(xs + ys);

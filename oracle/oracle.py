@@ -61,7 +61,7 @@ def lib():
         L.oracle_bin_sort.restype = _i64
         L.oracle_window_pixel.argtypes = [_i64, _i64, _i64, _i64]
         L.oracle_window_pixel.restype = _i64
-        L.oracle_compute_gcf_phase.argtypes = [_dp, _i64, ctypes.c_double, ctypes.c_double, _fp, _fp]
+        L.oracle_compute_gcf_phase.argtypes = [_dp, _i64, ctypes.c_double, ctypes.c_double, ctypes.c_int, _fp, _fp]
         L.oracle_grid_const_tap.argtypes = [_dp, _dp, _dp, _i64, _i64, _i64, _fp, _fp]
         _lib = L
     return _lib
@@ -224,11 +224,14 @@ def window_pixel(m, t, S, N):
     return int(lib().oracle_window_pixel(m, t, S, N))
 
 
-def compute_gcf_phase(uv, T, w):
+def compute_gcf_phase(uv, T, w, fixed_point=False):
+    """COMPUTE_GCF tap values (r2, i2) per visibility; fixed_point selects the
+    gather/moving-window form of dx."""
     uv = _c(uv, np.float64).reshape(-1, 2)
     r2 = np.empty(uv.shape[0], np.float32)
     i2 = np.empty(uv.shape[0], np.float32)
-    lib().oracle_compute_gcf_phase(_p(uv, _dp), uv.shape[0], float(T), float(w), _p(r2, _fp), _p(i2, _fp))
+    lib().oracle_compute_gcf_phase(_p(uv, _dp), uv.shape[0], float(T), float(w), int(fixed_point),
+                                   _p(r2, _fp), _p(i2, _fp))
     return r2, i2
 
 

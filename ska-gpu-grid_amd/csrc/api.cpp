@@ -129,9 +129,11 @@ static int grid_device(migrid_ctx* ctx, const migrid_job* job, const migrid_opts
         return fail(MIGRID_EINVAL, "in_type=%d", job->in_type);
     if (job->out_type != MIGRID_F64 && job->out_type != MIGRID_F32)
         return fail(MIGRID_EINVAL, "out_type=%d", job->out_type);
-    if (job->in_type == MIGRID_F64 && job->out_type == MIGRID_F32)
+    if (job->in_type == MIGRID_F64 && job->out_type == MIGRID_F32 &&
+        !(o.strategy == MIGRID_TILED && o.arith != MIGRID_ARITH_NATIVE))
         return fail(MIGRID_ENOTSUP, "double inputs with a float grid: the reference has no such "
-                                    "instantiation (grid_gpu.cu:820-825)");
+                                    "instantiation (grid_gpu.cu:820-825); the tiled strategy "
+                                    "offers it with FP32 arithmetic");
     if (!job->grid) return fail(MIGRID_EINVAL, "grid is NULL");
     if (job->npts > 0 && (!job->uv || !job->vals))
         return fail(MIGRID_EINVAL, "uv/vals NULL with npts=%lld", (long long)job->npts);

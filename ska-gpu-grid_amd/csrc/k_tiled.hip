@@ -1,9 +1,643 @@
-// Placeholder until the tiled gridder lands.
+// Tiled owner gridder: bin-sorted visibilities, sub-grid tiles held in
+// registers, sub-kernels staged in LDS, atomics only where a tile is flushed.
+//
+// This is the build's form of all three reference strategies
+// (scatter grid_gpu.cu:95-173, gather :391-503, moving window :504-673).
+// Gridding is a weighted scatter, not a dense contraction, so MFMA is not
+// used; the kernel is built for wave64 VALU + LDS:
+//
+//  * The grid is cut into T x T tiles.  A workgroup of WX x WY waves owns one
+//    tile; wave (wx, wy) owns an (8*RX) x (8*RY) block of it, and lane
+//    (lx, ly) = (lane & 7, lane >> 3) owns the RX x RY pixels
+//    (Xw + 8j + lx, Yw + 8i + ly), accumulated in registers.
+//  * Visibilities are bin-sorted by (k, bin_y, bin_x) over q = floor(uv)+S/2
+//    (k_bin.hip).  Pixel x receives visibilities with q_x in [x, x+S-1], so a
+//    tile's visibilities for one k are a few contiguous CSR ranges.
+//  * For each k the workgroup stages sub-kernel k into LDS with a zero border
+//    of 7 on every side.  For a visibility, an 8 x 8 register block is either
+//    untouched (skipped on a wave-uniform test) or reads its 64 coefficients
+//    with one ds_read per lane at an immediate offset; lanes outside the
+//    S x S window read the zero border, so no per-lane masks are needed.
+//  * A visibility record (q, value) is wave-uniform and comes through scalar
+//    loads; the complex multiply-accumulate is 4 FMAs per tap with the value
+//    in SGPRs.
+//  * Work items are (tile, k-range) pairs sized by visibility count, taken
+//    from a device queue by persistent workgroups.  A tile that is one item
+//    is flushed with a plain read-add-write; a split tile with hardware
+//    atomics (FP64, FP32, or FP32 with a two-sum compensation grid).
+//
+// Tap semantics as the reference (grid.cu:115-132): tap (r, c) of a
+// visibility with q = main + S/2 lands on pixel (q_x - S + 1 + c,
+// q_y - S + 1 + r) with coefficient G[k][r][c].
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "bin.hpp"
+#include "device_common.hpp"
 #include "internal.hpp"
+#include "tiled.hpp"
 
 namespace migrid {
-int launch_tiled(migrid_ctx*, const GridArgs&, const migrid_opts& o, hipStream_t)
+namespace {
+
+constexpr int kPad = 7;  // zero border of a staged sub-kernel
+
+// ---- staged coefficient types ------------------------------------------------
+struct Bf16x2 {
+    uint32_t w;
+};
+
+__device__ __forceinline__ uint32_t f32_to_bf16_rne(float f)
 {
-    return fail(MIGRID_ENOTSUP, "strategy %d not built yet", o.strategy);
+    uint32_t b = __float_as_uint(f);
+    b += 0x7fffu + ((b >> 16) & 1u);
+    return b >> 16;
 }
+
+template <typename G>
+struct Stage;
+template <>
+struct Stage<double2> {
+    using P = double;
+    __device__ static double2 make(double r, double i) { return make_double2(r, i); }
+    __device__ static void get(const double2& g, double& r, double& i)
+    {
+        r = g.x;
+        i = g.y;
+    }
+};
+template <>
+struct Stage<float2> {
+    using P = float;
+    __device__ static float2 make(double r, double i) { return make_float2((float)r, (float)i); }
+    __device__ static void get(const float2& g, float& r, float& i)
+    {
+        r = g.x;
+        i = g.y;
+    }
+};
+template <>
+struct Stage<Bf16x2> {
+    using P = float;
+    __device__ static Bf16x2 make(double r, double i)
+    {
+        Bf16x2 b;
+        b.w = f32_to_bf16_rne((float)r) | (f32_to_bf16_rne((float)i) << 16);
+        return b;
+    }
+    __device__ static void get(const Bf16x2& g, float& r, float& i)
+    {
+        r = __uint_as_float(g.w << 16);
+        i = __uint_as_float(g.w & 0xffff0000u);
+    }
+};
+
+template <typename T>
+struct Cplx {
+    T re, im;
+};
+
+// ---- packing ------------------------------------------------------------------
+// Record value transforms.
+constexpr int kValPlain = 0;    // the visibility value
+constexpr int kValCompatS = 1;  // value x COMPUTE_GCF tap value, scatter form
+constexpr int kValCompatI = 2;  // the same with the fixed-point dx of gather/window
+constexpr int kValDebug = 3;    // (1, n): the DEBUG1 coverage map
+
+template <typename TI, typename R, int VT>
+__global__ __launch_bounds__(256) void k_pack(const TI* __restrict__ uv, const TI* __restrict__ vals,
+                                              const int32_t* __restrict__ perm,
+                                              const int32_t* __restrict__ count_p, int S,
+                                              const TI* __restrict__ gcf, R* __restrict__ rec)
+{
+    const int64_t count = *count_p;
+    float p1 = 0.f, p2 = 0.f;
+    if (VT == kValCompatS || VT == kValCompatI) {
+        const int64_t e = (int64_t)S * (S - 1) / 2 - 1;  // grid_gpu.cu:104-107
+        const double T = (double)gcf[2 * e], w = (double)gcf[2 * e + 1];
+        p1 = (float)(2 * 3.1415926 * w);
+        p2 = (float)(p1 * T);
+    }
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < count;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const int n = perm[j];
+        const TI u = uv[2 * n], v = uv[2 * n + 1];
+        const TI fu = floor_t(u), fv = floor_t(v);
+        const int mx = (int)fu, my = (int)fv;
+        const uint32_t qx = (uint32_t)(mx + S / 2), qy = (uint32_t)(my + S / 2);
+        TI vr = vals[2 * n], vi = vals[2 * n + 1];
+        if (VT == kValCompatS || VT == kValCompatI) {
+            const int sx = (int)floor_t((TI)kOversample * (u - fu));
+            const int sy = (int)floor_t((TI)kOversample * (v - fv));
+            float xs, ys;
+            if (VT == kValCompatS) {  // grid_gpu.cu:141-142, float uv
+                xs = (float)(mx - u + sx * 1.0 / 8.0);
+                ys = (float)(my - v + sy * 1.0 / 8.0);
+            } else {  // grid_gpu.cu:456-459, int2 fixed point inn = main*8+sub
+                const int ix = mx * kOversample + sx, iy = my * kOversample + sy;
+                xs = (float)(mx - ix + sx * 1.0 / kOversample);
+                ys = (float)(my - iy + sy * 1.0 / kOversample);
+            }
+            xs *= xs;
+            ys *= ys;
+            const float ph = p1 - p2 * sqrtf(xs + ys);
+            float s_, c_;
+            sincosf(ph, &s_, &c_);
+            const TI r2 = (TI)s_, i2 = (TI)c_;
+            const TI nr = vr * r2 - vi * i2;
+            const TI ni = vr * i2 + r2 * vi;
+            vr = nr;
+            vi = ni;
+        } else if (VT == kValDebug) {
+            vr = (TI)1;
+            vi = (TI)n;
+        }
+        R r;
+        r.q = qx | (qy << 16);
+        r.n = (uint32_t)n;
+        r.vr = (decltype(r.vr))vr;
+        r.vi = (decltype(r.vi))vi;
+        rec[j] = r;
+    }
+}
+
+// ---- work items -----------------------------------------------------------------
+// One wave per tile; lane = sub-kernel k.  Counts the tile's visibilities per
+// k from the CSR offsets and cuts k into items of about `target` visibilities
+// (target <= 0: one item per tile).
+__global__ __launch_bounds__(256) void k_items(const int32_t* __restrict__ offs, TileGeom g,
+                                               int64_t target, Item* __restrict__ items,
+                                               QueueState* __restrict__ qs,
+                                               uint8_t* __restrict__ split)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t ntiles = (int64_t)g.ntx * g.ntx;
+    if (tile >= ntiles) return;
+    const int tx = (int)(tile % g.ntx), ty = (int)(tile / g.ntx);
+    const int bxl = tx * g.T / g.H, byl = ty * g.T / g.H;
+    const int bxh = min(g.nb - 1, (tx * g.T + g.T + g.S - 2) / g.H);
+    const int byh = min(g.nb - 1, (ty * g.T + g.T + g.S - 2) / g.H);
+    const int k = lane;
+    int64_t cnt = 0;
+    for (int by = byl; by <= byh; by++) {
+        const int64_t key = ((int64_t)k * g.nb + by) * g.nb;
+        cnt += offs[key + bxh + 1] - offs[key + bxl];
+    }
+    // inclusive scan over the 64 lanes
+    int64_t inc = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    const int64_t total = __shfl(inc, 63, 64);
+    if (total == 0) return;
+    int nit = 1;
+    if (target > 0) {
+        int64_t want = (total + target - 1) / target;
+        nit = (int)(want < 64 ? want : 64);
+    }
+    // item index of each non-empty k; an item starts at a non-empty k whose
+    // index differs from that of the previous non-empty k, so no item is empty
+    const int64_t excl = inc - cnt;
+    const int j = (int)(excl * nit / total);
+    const uint64_t nz = __ballot(cnt > 0);
+    const uint64_t below = nz & ((1ull << lane) - 1);
+    const int prev = below ? 63 - __builtin_clzll(below) : -1;
+    const int jprev = __shfl(j, prev < 0 ? 0 : prev, 64);
+    const bool start = cnt > 0 && (prev < 0 || j != jprev);
+    const uint64_t m = __ballot(start);
+    const int n_items = __popcll(m);
+    int base = 0;
+    if (lane == 0) base = atomicAdd(&qs->nitems, n_items);
+    base = __shfl(base, 0, 64);
+    if (start) {
+        const int slot = base + __popcll(m & ((1ull << lane) - 1));
+        const uint64_t above = lane == 63 ? 0 : (m >> (lane + 1));
+        const int khi = above ? lane + __builtin_ctzll(above) : 63;
+        Item it;
+        it.tile = (int32_t)tile;
+        it.klo = (int16_t)lane;
+        it.khi = (int16_t)khi;
+        items[slot] = it;
+    }
+    if (lane == 0) split[tile] = n_items > 1;
+}
+
+// ---- flush helpers --------------------------------------------------------------
+template <typename TO, typename TA>
+__device__ __forceinline__ void flush_px(TO* grid, TO* lo, int64_t idx_re, int64_t idx_im, TA re,
+                                         TA im, bool atomic)
+{
+    if (lo != nullptr) {
+        // FP32 grid with a two-sum compensation grid (Knuth's TwoSum on the
+        // value the hardware added to): grid + lo carries the rounding error
+        const float t[2] = {(float)re, (float)im};
+        const int64_t ix[2] = {idx_re, idx_im};
+        for (int c = 0; c < 2; c++) {
+            float* hp = (float*)grid + ix[c];
+            float a = atomic ? atomicAdd(hp, t[c]) : *hp;
+            const float s = a + t[c];
+            if (!atomic) *hp = s;
+            const float bb = s - a;
+            const float err = (a - (s - bb)) + (t[c] - bb);
+            float* lp = (float*)lo + ix[c];
+            if (atomic)
+                atomic_add(lp, err);
+            else
+                *lp += err;
+        }
+        return;
+    }
+    if (atomic) {
+        atomic_add(grid + idx_re, (TO)re);
+        atomic_add(grid + idx_im, (TO)im);
+    } else {
+        grid[idx_re] += (TO)re;
+        grid[idx_im] += (TO)im;
+    }
+}
+
+// ---- the owner kernel -----------------------------------------------------------
+// G: staged coefficient type; TA: accumulator type; R: record type;
+// TI: type of the gcf table in HBM; TO: grid type.
+// GM: 0 table, 1 ones (COMPAT/DEBUG records carry the tap value), 2 analytic.
+template <int RX, int RY, int WX, int WY, typename G, typename TA, typename R, typename TI,
+          typename TO, bool PLANAR, int GM>
+__global__ __launch_bounds__(64 * WX * WY) void k_owner(
+    const R* __restrict__ rec, const int32_t* __restrict__ offs, const Item* __restrict__ items,
+    QueueState* __restrict__ qs, const uint8_t* __restrict__ split, const TI* __restrict__ gcf,
+    TO* __restrict__ grid, TO* __restrict__ grid_lo, TileGeom g, double sigma, double beta)
+{
+    using P = typename Stage<G>::P;  // product type
+    constexpr int NT = 64 * WX * WY;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    G* sg = (G*)smem_raw;
+    __shared__ int s_item;
+    __shared__ Cplx<double> s_fac[2][64];  // analytic separable factors
+
+    const int S = g.S;
+    const int pitch = S + 2 * kPad;
+    const int rows = S + 2 * kPad;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const int wx = w % WX, wy = w / WX;
+    const int lx = lane & 7, ly = lane >> 3;
+    const int lane_off = (ly + kPad) * pitch + lx + kPad;
+    const int64_t NN = (int64_t)g.N * g.N;
+
+    // zero the whole staging buffer once: the border stays zero
+    for (int e = tid; e < rows * pitch; e += NT) sg[e] = Stage<G>::make(0.0, 0.0);
+    if (GM == 1) {
+        for (int e = tid; e < S * S; e += NT)
+            sg[(e / S + kPad) * pitch + e % S + kPad] = Stage<G>::make(1.0, 0.0);
+    }
+    __syncthreads();
+
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(&qs->head, 1);
+        __syncthreads();
+        const int it_idx = s_item;
+        if (it_idx >= qs->nitems) break;
+        const Item it = items[it_idx];
+        const int tile = __builtin_amdgcn_readfirstlane(it.tile);
+        const int klo = __builtin_amdgcn_readfirstlane(it.klo);
+        const int khi = __builtin_amdgcn_readfirstlane(it.khi);
+        const int tx = tile % g.ntx, ty = tile / g.ntx;
+        const int X0 = tx * g.T, Y0 = ty * g.T;
+        const int Xw = X0 + wx * 8 * RX, Yw = Y0 + wy * 8 * RY;
+        const int bxl = X0 / g.H, byl = Y0 / g.H;
+        const int bxh = min(g.nb - 1, (X0 + g.T + S - 2) / g.H);
+        const int byh = min(g.nb - 1, (Y0 + g.T + S - 2) / g.H);
+
+        Cplx<TA> acc[RY][RX];
+#pragma unroll
+        for (int i = 0; i < RY; i++)
+#pragma unroll
+            for (int j = 0; j < RX; j++) acc[i][j].re = acc[i][j].im = (TA)0;
+
+        for (int k = klo; k <= khi; k++) {
+            int64_t nk = 0;
+            for (int by = byl; by <= byh; by++) {
+                const int64_t key = ((int64_t)k * g.nb + by) * g.nb;
+                nk += offs[key + bxh + 1] - offs[key + bxl];
+            }
+            if (nk == 0) continue;
+            if (GM != 1) {
+                __syncthreads();  // previous sub-kernel no longer read
+                if (GM == 0) {
+                    const TI* gk = gcf + 2 * (int64_t)k * S * S;
+                    for (int e = tid; e < S * S; e += NT)
+                        sg[(e / S + kPad) * pitch + e % S + kPad] =
+                            Stage<G>::make((double)gk[2 * e], (double)gk[2 * e + 1]);
+                } else {
+                    // separable analytic GCF: G[r][c] = f(dy_r) * f(dx_c),
+                    // f(d) = exp(-d^2/(2 sigma^2)) * exp(i beta d^2)
+                    const int sx = k % kOversample, sy = k / kOversample;
+                    for (int e = tid; e < 2 * S; e += NT) {
+                        const int axis = e / S, t = e % S;
+                        const double d = (double)(t - S / 2 + 1) - (axis ? sy : sx) / 8.0;
+                        const double q = d * d;
+                        const double amp = exp(-q / (2.0 * sigma * sigma));
+                        double sn, cs;
+                        sincos(beta * q, &sn, &cs);
+                        s_fac[axis][t].re = amp * cs;
+                        s_fac[axis][t].im = amp * sn;
+                    }
+                    __syncthreads();
+                    for (int e = tid; e < S * S; e += NT) {
+                        const int r = e / S, c = e % S;
+                        const Cplx<double> a = s_fac[1][r], b = s_fac[0][c];
+                        sg[(r + kPad) * pitch + c + kPad] =
+                            Stage<G>::make(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+                    }
+                }
+                __syncthreads();
+            }
+            for (int by = byl; by <= byh; by++) {
+                const int64_t key = ((int64_t)k * g.nb + by) * g.nb;
+                const int r0 = __builtin_amdgcn_readfirstlane(offs[key + bxl]);
+                const int r1 = __builtin_amdgcn_readfirstlane(offs[key + bxh + 1]);
+                for (int r = r0; r < r1; r++) {
+                    const R vrec = rec[r];
+                    const int qx = (int)(vrec.q & 0xffffu), qy = (int)(vrec.q >> 16);
+                    // column/row of the staged kernel read by lane (0,0), block (0,0)
+                    const int c0 = Xw - qx + S - 1;
+                    const int rr0 = Yw - qy + S - 1;
+                    // touched blocks: c0 + 8j in [-7, S-1]
+                    const int jlo = max(0, (c0 < -kPad) ? (-kPad - c0 + 7) >> 3 : 0);
+                    const int jhi = min(RX - 1, (S - 1 - c0) >= 0 ? (S - 1 - c0) >> 3 : -1);
+                    const int ilo = max(0, (rr0 < -kPad) ? (-kPad - rr0 + 7) >> 3 : 0);
+                    const int ihi = min(RY - 1, (S - 1 - rr0) >= 0 ? (S - 1 - rr0) >> 3 : -1);
+                    if (jlo > jhi || ilo > ihi) continue;
+                    const P vr = (P)vrec.vr, vi = (P)vrec.vi;
+                    const G* base = sg + lane_off + rr0 * pitch + c0;
+#pragma unroll
+                    for (int i = 0; i < RY; i++) {
+                        if (i < ilo || i > ihi) continue;
+#pragma unroll
+                        for (int j = 0; j < RX; j++) {
+                            if (j < jlo || j > jhi) continue;
+                            P gr, gi;
+                            Stage<G>::get(base[8 * i * pitch + 8 * j], gr, gi);
+                            if (sizeof(P) == sizeof(TA)) {
+                                acc[i][j].re = fma((TA)vr, (TA)gr, acc[i][j].re);
+                                acc[i][j].re = fma(-(TA)vi, (TA)gi, acc[i][j].re);
+                                acc[i][j].im = fma((TA)vr, (TA)gi, acc[i][j].im);
+                                acc[i][j].im = fma((TA)vi, (TA)gr, acc[i][j].im);
+                            } else {  // float products summed in double
+                                acc[i][j].re += (TA)(vr * gr - vi * gi);
+                                acc[i][j].im += (TA)(vr * gi + gr * vi);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // flush
+        const bool atomic = split[tile] != 0;
+#pragma unroll
+        for (int i = 0; i < RY; i++) {
+            const int y = Yw + 8 * i + ly;
+            if (y >= g.N) continue;
+#pragma unroll
+            for (int j = 0; j < RX; j++) {
+                const int x = Xw + 8 * j + lx;
+                if (x >= g.N) continue;
+                const int64_t p = (int64_t)y * g.N + x;
+                if (PLANAR)
+                    flush_px(grid, grid_lo, p, NN + p, acc[i][j].re, acc[i][j].im, atomic);
+                else
+                    flush_px(grid, grid_lo, 2 * p, 2 * p + 1, acc[i][j].re, acc[i][j].im, atomic);
+            }
+        }
+        __syncthreads();  // s_item reuse
+    }
+}
+
+// ---- host side ------------------------------------------------------------------
+struct TileCfg {
+    int T;
+    int H;
+};
+
+TileCfg pick_tile(int S, int tile_opt)
+{
+    TileCfg c;
+    c.T = tile_opt > 0 ? tile_opt : (S >= 48 ? 64 : 32);
+    // bin edge: the largest power of two <= max(S, 8) that divides T
+    int h = 8;
+    while (h * 2 <= S && h * 2 <= c.T) h *= 2;
+    c.H = h;
+    while (c.T % c.H) c.H /= 2;
+    return c;
+}
+
+struct Launch {
+    const void* rec;
+    const int32_t* offs;
+    const Item* items;
+    QueueState* qs;
+    const uint8_t* split;
+    const void* gcf;
+    void* grid;
+    void* grid_lo;
+    TileGeom g;
+    double sigma, beta;
+    size_t lds;
+    int blocks;
+};
+
+template <int RX, int RY, int WX, int WY, typename G, typename TA, typename R, typename TI,
+          typename TO, bool PLANAR, int GM>
+int run_owner(const Launch& L, hipStream_t s)
+{
+    auto kern = k_owner<RX, RY, WX, WY, G, TA, R, TI, TO, PLANAR, GM>;
+    MG_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)L.lds));
+    int per_cu = 0;
+    MG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * WX * WY, L.lds));
+    if (per_cu < 1) return fail(MIGRID_ENOTSUP, "tiled: kernel does not fit (lds=%zu)", L.lds);
+    int dev = 0, ncu = 0;
+    MG_HIP(hipGetDevice(&dev));
+    MG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int blocks = ncu * per_cu;
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * WX * WY), L.lds, s, (const R*)L.rec, L.offs,
+                       L.items, L.qs, L.split, (const TI*)L.gcf, (TO*)L.grid, (TO*)L.grid_lo, L.g,
+                       L.sigma, L.beta);
+    MG_HIP(hipGetLastError());
+    return MIGRID_OK;
+}
+
+// Shape dispatch: T = 64 uses 2x2 waves of 4x4 blocks, T = 32 2x2 waves of 2x2.
+template <typename G, typename TA, typename R, typename TI, typename TO, bool PLANAR, int GM>
+int run_shape(const Launch& L, hipStream_t s)
+{
+    if (L.g.T == 64) return run_owner<4, 4, 2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);
+    if (L.g.T == 32) return run_owner<2, 2, 2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);
+    return fail(MIGRID_ENOTSUP, "tiled: tile edge %d (32 or 64 supported)", L.g.T);
+}
+
+template <typename G, typename TA, typename R, typename TI, typename TO>
+int run_modes(const Launch& L, bool planar, int gm, hipStream_t s)
+{
+    if (planar) {
+        if (gm == 0) return run_shape<G, TA, R, TI, TO, true, 0>(L, s);
+        if (gm == 1) return run_shape<G, TA, R, TI, TO, true, 1>(L, s);
+        return run_shape<G, TA, R, TI, TO, true, 2>(L, s);
+    }
+    if (gm == 0) return run_shape<G, TA, R, TI, TO, false, 0>(L, s);
+    if (gm == 1) return run_shape<G, TA, R, TI, TO, false, 1>(L, s);
+    return run_shape<G, TA, R, TI, TO, false, 2>(L, s);
+}
+
+template <typename TI, typename R>
+void launch_pack(const GridArgs& a, int vt, const int32_t* perm, const int32_t* count, R* rec,
+                 hipStream_t s)
+{
+    const int blocks = (int)std::min<int64_t>(4096, (a.npts + 255) / 256 + 1);
+    switch (vt) {
+    case kValCompatS:
+        hipLaunchKernelGGL((k_pack<TI, R, kValCompatS>), dim3(blocks), dim3(256), 0, s,
+                           (const TI*)a.uv, (const TI*)a.vals, perm, count, a.S, (const TI*)a.gcf, rec);
+        break;
+    case kValCompatI:
+        hipLaunchKernelGGL((k_pack<TI, R, kValCompatI>), dim3(blocks), dim3(256), 0, s,
+                           (const TI*)a.uv, (const TI*)a.vals, perm, count, a.S, (const TI*)a.gcf, rec);
+        break;
+    case kValDebug:
+        hipLaunchKernelGGL((k_pack<TI, R, kValDebug>), dim3(blocks), dim3(256), 0, s,
+                           (const TI*)a.uv, (const TI*)a.vals, perm, count, a.S, (const TI*)a.gcf, rec);
+        break;
+    default:
+        hipLaunchKernelGGL((k_pack<TI, R, kValPlain>), dim3(blocks), dim3(256), 0, s,
+                           (const TI*)a.uv, (const TI*)a.vals, perm, count, a.S, (const TI*)a.gcf, rec);
+    }
+}
+
+}  // namespace
+
+int launch_tiled(migrid_ctx* ctx, const GridArgs& a, const migrid_opts& o, hipStream_t s)
+{
+    if (!ctx) return fail(MIGRID_EINVAL, "tiled strategies need a context");
+    if (a.S > 64) return fail(MIGRID_ENOTSUP, "tiled strategies support S <= 64 (S=%d)", a.S);
+    if (a.N + a.S - 2 >= 65536) return fail(MIGRID_ENOTSUP, "tiled: N + S - 2 must be < 65536");
+    const bool f32arith = o.arith == MIGRID_ARITH_F32 || o.arith == MIGRID_ARITH_F32_BF16GCF;
+    const bool bf16 = o.arith == MIGRID_ARITH_F32_BF16GCF;
+    if (o.strategy != MIGRID_TILED && o.arith != MIGRID_ARITH_NATIVE)
+        return fail(MIGRID_ENOTSUP, "gather/window run native arithmetic only");
+    if (a.debug && (f32arith || a.out_type != MIGRID_F64))
+        return fail(MIGRID_ENOTSUP, "coverage mode needs native arithmetic into a double grid");
+    if (a.grid_lo && !(a.out_type == MIGRID_F32))
+        return fail(MIGRID_ENOTSUP, "a compensation grid needs a float grid");
+    if (a.npts == 0) return MIGRID_OK;
+
+    const TileCfg tc = pick_tile(a.S, o.tile);
+    const BinGeom bg = bin_geom(a.N, a.S, tc.H);
+    if (bg.nkeys >= ((int64_t)1 << 31)) return fail(MIGRID_ENOTSUP, "tiled: too many bins");
+    TileGeom g;
+    g.N = a.N;
+    g.S = a.S;
+    g.T = tc.T;
+    g.H = tc.H;
+    g.nb = bg.nb;
+    g.ntx = (a.N + tc.T - 1) / tc.T;
+    const int64_t ntiles = (int64_t)g.ntx * g.ntx;
+
+    // records: 16 B (float values) or 32 B (double values)
+    const bool rec64 = !f32arith && a.in_type == MIGRID_F64;
+    const size_t rec_bytes = (size_t)a.npts * (rec64 ? sizeof(RecF64) : sizeof(RecF32));
+    const size_t perm_bytes = ((size_t)a.npts * 4 + 255) & ~(size_t)255;
+    const size_t offs_bytes = (((size_t)bg.nkeys + 1) * 4 + 255) & ~(size_t)255;
+    const size_t item_cap = (size_t)ntiles * kSubKernels;
+    const size_t items_bytes = (item_cap * sizeof(Item) + 255) & ~(size_t)255;
+    const size_t split_bytes = ((size_t)ntiles + 255) & ~(size_t)255;
+    MG_TRY(ctx->bins.ensure(bin_scratch_bytes(a.npts, bg)));
+    MG_TRY(ctx->records.ensure(rec_bytes + perm_bytes + offs_bytes));
+    MG_TRY(ctx->queue.ensure(items_bytes + split_bytes + sizeof(QueueState)));
+    char* rp = (char*)ctx->records.p;
+    void* rec = rp;
+    int32_t* perm = (int32_t*)(rp + ((rec_bytes + 255) & ~(size_t)255));
+    int32_t* offs = (int32_t*)((char*)perm + perm_bytes);
+    // (records region sized with slack for the alignment above)
+    Item* items = (Item*)ctx->queue.p;
+    uint8_t* split = (uint8_t*)ctx->queue.p + items_bytes;
+    QueueState* qs = (QueueState*)((char*)ctx->queue.p + items_bytes + split_bytes);
+
+    MG_TRY(bin_sort(bg, a.in_type, a.uv, a.npts, ctx->bins.p, perm, offs, s));
+
+    int vt = kValPlain;
+    if (a.debug)
+        vt = kValDebug;
+    else if (a.gcf_mode == MIGRID_GCF_COMPAT)
+        vt = o.strategy == MIGRID_TILED ? kValCompatS : kValCompatI;
+    const int32_t* count = offs + bg.nkeys;
+    if (a.in_type == MIGRID_F64) {
+        if (rec64)
+            launch_pack<double, RecF64>(a, vt, perm, count, (RecF64*)rec, s);
+        else
+            launch_pack<double, RecF32>(a, vt, perm, count, (RecF32*)rec, s);
+    } else {
+        launch_pack<float, RecF32>(a, vt, perm, count, (RecF32*)rec, s);
+    }
+    MG_HIP(hipGetLastError());
+
+    MG_HIP(hipMemsetAsync(qs, 0, sizeof(QueueState), s));
+    const int64_t target = o.strategy == MIGRID_TILED ? (o.vis_per_wave > 0 ? o.vis_per_wave : 4096) : 0;
+    hipLaunchKernelGGL(k_items, dim3((unsigned)((ntiles + 3) / 4)), dim3(256), 0, s, offs, g, target,
+                       items, qs, split);
+    MG_HIP(hipGetLastError());
+
+    Launch L;
+    L.rec = rec;
+    L.offs = offs;
+    L.items = items;
+    L.qs = qs;
+    L.split = split;
+    L.gcf = a.gcf;
+    L.grid = a.grid;
+    L.grid_lo = a.grid_lo;
+    L.g = g;
+    L.sigma = a.sigma;
+    L.beta = a.beta;
+    const int gm = (a.debug || a.gcf_mode == MIGRID_GCF_COMPAT) ? 1
+                   : a.gcf_mode == MIGRID_GCF_ANALYTIC     ? 2
+                                                           : 0;
+    const size_t cells = (size_t)(a.S + 2 * kPad) * (a.S + 2 * kPad);
+    MG_TRY(ctx->prof.begin(s));
+    int rc;
+    const bool in64 = a.in_type == MIGRID_F64, out64 = a.out_type == MIGRID_F64;
+    if (bf16) {
+        L.lds = cells * sizeof(Bf16x2);
+        if (out64)
+            rc = in64 ? run_modes<Bf16x2, float, RecF32, double, double>(L, a.planar, gm, s)
+                      : run_modes<Bf16x2, float, RecF32, float, double>(L, a.planar, gm, s);
+        else
+            rc = in64 ? run_modes<Bf16x2, float, RecF32, double, float>(L, a.planar, gm, s)
+                      : run_modes<Bf16x2, float, RecF32, float, float>(L, a.planar, gm, s);
+    } else if (f32arith) {
+        L.lds = cells * sizeof(float2);
+        if (out64)
+            rc = in64 ? run_modes<float2, float, RecF32, double, double>(L, a.planar, gm, s)
+                      : run_modes<float2, float, RecF32, float, double>(L, a.planar, gm, s);
+        else
+            rc = in64 ? run_modes<float2, float, RecF32, double, float>(L, a.planar, gm, s)
+                      : run_modes<float2, float, RecF32, float, float>(L, a.planar, gm, s);
+    } else if (a.in_type == MIGRID_F64) {  // zz
+        L.lds = cells * sizeof(double2);
+        rc = run_modes<double2, double, RecF64, double, double>(L, a.planar, gm, s);
+    } else if (a.out_type == MIGRID_F64) {  // zc: float products, double sums
+        L.lds = cells * sizeof(float2);
+        rc = run_modes<float2, double, RecF32, float, double>(L, a.planar, gm, s);
+    } else {  // cc
+        L.lds = cells * sizeof(float2);
+        rc = run_modes<float2, float, RecF32, float, float>(L, a.planar, gm, s);
+    }
+    if (rc != MIGRID_OK) return rc;
+    MG_TRY(ctx->prof.end(s));
+    return MIGRID_OK;
+}
+
 }  // namespace migrid
