@@ -104,12 +104,6 @@ void Profiler::release()
     used = 0;
 }
 
-int launch_zero(void* p, size_t bytes, hipStream_t s)
-{
-    if (bytes) MG_HIP(hipMemsetAsync(p, 0, bytes, s));
-    return MIGRID_OK;
-}
-
 int launch_tiled(migrid_ctx* ctx, const GridArgs& a, const migrid_opts& o, hipStream_t s);
 
 static size_t elem_bytes(int t) { return t == MIGRID_F64 ? 8 : 4; }

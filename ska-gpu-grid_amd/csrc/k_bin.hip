@@ -226,7 +226,7 @@ int exclusive_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* partial,
                    hipStream_t s)
 {
     if (n <= 0) {
-        if (write_total) MG_HIP(hipMemsetAsync(out, 0, sizeof(int32_t), s));
+        if (write_total) MG_TRY(fill_u32(out, 0u, 1, s));
         return MIGRID_OK;
     }
     const int64_t nblk = (n + kScanTile - 1) / kScanTile;
@@ -281,7 +281,7 @@ int bin_sort(const BinGeom& g, int in_type, const void* uv, int64_t npts, void* 
     const int64_t big = g.nkeys + 1 > 256 * (nblk + 1) ? g.nkeys + 1 : 256 * (nblk + 1);
     int32_t* partial = (int32_t*)take((size_t)scan_scratch_elems(big) * 4);
 
-    MG_HIP(hipMemsetAsync(hist, 0, (size_t)(g.nkeys + 1) * 4, s));
+    MG_TRY(fill_u32(hist, 0u, g.nkeys + 1, s));
     if (npts > 0) {
         int blocks = (int)((npts + 255) / 256 < 4096 ? (npts + 255) / 256 : 4096);
         if (in_type == MIGRID_F64)
@@ -298,6 +298,7 @@ int bin_sort(const BinGeom& g, int in_type, const void* uv, int64_t npts, void* 
         uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
         for (int pass = 0; pass < passes; pass++) {
             const int shift = 8 * pass;
+            if (pass == passes - 1) vo = (uint32_t*)perm;  // last pass writes the permutation
             hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(kRsThreads), 0, s, ki, npts, shift,
                                counts);
             MG_TRY(exclusive_scan(counts, 256 * nblk, coffs, partial, false, s));
@@ -311,8 +312,7 @@ int bin_sort(const BinGeom& g, int in_type, const void* uv, int64_t npts, void* 
             vi = vo;
             vo = tv;
         }
-        // sorted indices of binned visibilities come first (sentinels sort last)
-        MG_HIP(hipMemcpyAsync(perm, vi, (size_t)npts * 4, hipMemcpyDeviceToDevice, s));
+        // perm: sorted indices of binned visibilities first (sentinels sort last)
     }
     MG_TRY(exclusive_scan(hist, g.nkeys, offsets, partial, true, s));
     return MIGRID_OK;

@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_pack(const TI* __restrict__ uv, const T
 // k from the CSR offsets and cuts k into items of about `target` visibilities
 // (target <= 0: one item per tile).
 __global__ __launch_bounds__(256) void k_items(const int32_t* __restrict__ offs, TileGeom g,
-                                               int64_t target, Item* __restrict__ items,
+                                               int64_t target, int job, Item* __restrict__ items,
                                                QueueState* __restrict__ qs,
                                                uint8_t* __restrict__ split)
 {
@@ -220,6 +220,8 @@ __global__ __launch_bounds__(256) void k_items(const int32_t* __restrict__ offs,
         it.tile = (int32_t)tile;
         it.klo = (int16_t)lane;
         it.khi = (int16_t)khi;
+        it.job = job;
+        it.pad = 0;
         items[slot] = it;
     }
     if (lane == 0) split[tile] = n_items > 1;
@@ -265,10 +267,9 @@ __device__ __forceinline__ void flush_px(TO* grid, TO* lo, int64_t idx_re, int64
 // GM: 0 table, 1 ones (COMPAT/DEBUG records carry the tap value), 2 analytic.
 template <int RX, int RY, int WX, int WY, typename G, typename TA, typename R, typename TI,
           typename TO, bool PLANAR, int GM>
-__global__ __launch_bounds__(64 * WX * WY) void k_owner(
-    const R* __restrict__ rec, const int32_t* __restrict__ offs, const Item* __restrict__ items,
-    QueueState* __restrict__ qs, const uint8_t* __restrict__ split, const TI* __restrict__ gcf,
-    TO* __restrict__ grid, TO* __restrict__ grid_lo, TileGeom g, double sigma, double beta)
+__global__ __launch_bounds__(64 * WX * WY) void k_owner(const JobDesc* __restrict__ jobs,
+                                                        const Item* __restrict__ items,
+                                                        QueueState* __restrict__ qs, int smax)
 {
     using P = typename Stage<G>::P;  // product type
     constexpr int NT = 64 * WX * WY;
@@ -277,24 +278,17 @@ __global__ __launch_bounds__(64 * WX * WY) void k_owner(
     __shared__ int s_item;
     __shared__ Cplx<double> s_fac[2][64];  // analytic separable factors
 
-    const int S = g.S;
-    const int pitch = S + 2 * kPad;
-    const int rows = S + 2 * kPad;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
     const int wx = w % WX, wy = w / WX;
     const int lx = lane & 7, ly = lane >> 3;
-    const int lane_off = (ly + kPad) * pitch + lx + kPad;
-    const int64_t NN = (int64_t)g.N * g.N;
 
-    // zero the whole staging buffer once: the border stays zero
-    for (int e = tid; e < rows * pitch; e += NT) sg[e] = Stage<G>::make(0.0, 0.0);
-    if (GM == 1) {
-        for (int e = tid; e < S * S; e += NT)
-            sg[(e / S + kPad) * pitch + e % S + kPad] = Stage<G>::make(1.0, 0.0);
-    }
-    __syncthreads();
+    // zero the whole staging buffer once: the border stays zero as long as
+    // the support (hence the staged layout) does not change between items
+    for (int e = tid; e < (smax + 2 * kPad) * (smax + 2 * kPad); e += NT)
+        sg[e] = Stage<G>::make(0.0, 0.0);
+    int staged_s = -1;  // support of the layout now in LDS
 
     for (;;) {
         if (tid == 0) s_item = atomicAdd(&qs->head, 1);
@@ -305,6 +299,36 @@ __global__ __launch_bounds__(64 * WX * WY) void k_owner(
         const int tile = __builtin_amdgcn_readfirstlane(it.tile);
         const int klo = __builtin_amdgcn_readfirstlane(it.klo);
         const int khi = __builtin_amdgcn_readfirstlane(it.khi);
+        const int job = __builtin_amdgcn_readfirstlane(it.job);
+        const TileGeom g = jobs[job].g;
+        const R* __restrict__ rec = (const R*)jobs[job].rec;
+        const int32_t* __restrict__ offs = jobs[job].offs;
+        const TI* __restrict__ gcf = (const TI*)jobs[job].gcf;
+        TO* __restrict__ grid = (TO*)jobs[job].grid;
+        TO* __restrict__ grid_lo = (TO*)jobs[job].grid_lo;
+        const double sigma = jobs[job].sigma, beta = jobs[job].beta;
+        const bool atomic = jobs[job].split[tile] != 0;
+        const int S = g.S;
+        const int pitch = S + 2 * kPad;
+        const int lane_off = (ly + kPad) * pitch + lx + kPad;
+        const int64_t NN = (int64_t)g.N * g.N;
+        if (S != staged_s) {
+            // a new layout: clear the old interior (border positions of the
+            // new layout may have been interior before), then, for the
+            // constant-tap form, write the all-ones kernel once
+            __syncthreads();
+            if (staged_s > 0) {
+                const int op = staged_s + 2 * kPad;
+                for (int e = tid; e < staged_s * staged_s; e += NT)
+                    sg[(e / staged_s + kPad) * op + e % staged_s + kPad] = Stage<G>::make(0.0, 0.0);
+                __syncthreads();
+            }
+            if (GM == 1)
+                for (int e = tid; e < S * S; e += NT)
+                    sg[(e / S + kPad) * pitch + e % S + kPad] = Stage<G>::make(1.0, 0.0);
+            staged_s = S;
+        }
+        __syncthreads();
         const int tx = tile % g.ntx, ty = tile / g.ntx;
         const int X0 = tx * g.T, Y0 = ty * g.T;
         const int Xw = X0 + wx * 8 * RX, Yw = Y0 + wy * 8 * RY;
@@ -397,7 +421,6 @@ __global__ __launch_bounds__(64 * WX * WY) void k_owner(
             }
         }
         // flush
-        const bool atomic = split[tile] != 0;
 #pragma unroll
         for (int i = 0; i < RY; i++) {
             const int y = Yw + 8 * i + ly;
@@ -417,11 +440,100 @@ __global__ __launch_bounds__(64 * WX * WY) void k_owner(
     }
 }
 
+// Writes one job descriptor (passed by value, so the launch owns a copy).
+__global__ void k_set_desc(JobDesc* dst, JobDesc d) { *dst = d; }
+
 // ---- host side ------------------------------------------------------------------
-struct TileCfg {
-    int T;
-    int H;
-};
+template <int RX, int RY, int WX, int WY, typename G, typename TA, typename R, typename TI,
+          typename TO, bool PLANAR, int GM>
+int run_owner(const OwnerLaunch& L, hipStream_t s)
+{
+    auto kern = k_owner<RX, RY, WX, WY, G, TA, R, TI, TO, PLANAR, GM>;
+    MG_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)L.lds));
+    int per_cu = 0;
+    MG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * WX * WY, L.lds));
+    if (per_cu < 1) return fail(MIGRID_ENOTSUP, "tiled: kernel does not fit (lds=%zu)", L.lds);
+    int dev = 0, ncu = 0;
+    MG_HIP(hipGetDevice(&dev));
+    MG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    hipLaunchKernelGGL(kern, dim3(ncu * per_cu), dim3(64 * WX * WY), L.lds, s, L.jobs, L.items,
+                       L.qs, L.smax);
+    MG_HIP(hipGetLastError());
+    return MIGRID_OK;
+}
+
+// Tile shapes: T = 64 uses 2x2 waves of 4x4 blocks (2x4 waves of 4x2 blocks
+// when the staged kernel is FP64, which allows one workgroup per CU);
+// T = 32 2x2 waves of 2x2 blocks; T = 16 2x2 waves of one block.
+template <typename G, typename TA, typename R, typename TI, typename TO, bool PLANAR, int GM>
+int run_shape(int T, const OwnerLaunch& L, hipStream_t s)
+{
+    if (T == 64) {
+        if (sizeof(G) == 16) return run_owner<4, 2, 2, 4, G, TA, R, TI, TO, PLANAR, GM>(L, s);
+        return run_owner<4, 4, 2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);
+    }
+    if (T == 32) return run_owner<2, 2, 2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);
+    if (T == 16) return run_owner<1, 1, 2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);
+    return fail(MIGRID_ENOTSUP, "tiled: tile edge %d (16, 32 or 64 supported)", T);
+}
+
+template <typename G, typename TA, typename R, typename TI, typename TO>
+int run_modes(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
+{
+    if (k.planar) {
+        if (k.gm == 0) return run_shape<G, TA, R, TI, TO, true, 0>(k.T, L, s);
+        if (k.gm == 1) return run_shape<G, TA, R, TI, TO, true, 1>(k.T, L, s);
+        return run_shape<G, TA, R, TI, TO, true, 2>(k.T, L, s);
+    }
+    if (k.gm == 0) return run_shape<G, TA, R, TI, TO, false, 0>(k.T, L, s);
+    if (k.gm == 1) return run_shape<G, TA, R, TI, TO, false, 1>(k.T, L, s);
+    return run_shape<G, TA, R, TI, TO, false, 2>(k.T, L, s);
+}
+
+template <typename TI, typename R>
+void launch_pack(const GridArgs& a, int vt, const int32_t* perm, const int32_t* count, R* rec,
+                 hipStream_t s)
+{
+    const int blocks = (int)std::min<int64_t>(4096, (a.npts + 255) / 256 + 1);
+#define MG_PACK(VT)                                                                          \
+    hipLaunchKernelGGL((k_pack<TI, R, VT>), dim3(blocks), dim3(256), 0, s, (const TI*)a.uv,  \
+                       (const TI*)a.vals, perm, count, a.S, (const TI*)a.gcf, rec)
+    switch (vt) {
+    case kValCompatS: MG_PACK(kValCompatS); break;
+    case kValCompatI: MG_PACK(kValCompatI); break;
+    case kValDebug: MG_PACK(kValDebug); break;
+    default: MG_PACK(kValPlain);
+    }
+#undef MG_PACK
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+int dispatch_owner(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
+{
+    const bool in64 = k.in_type == MIGRID_F64, out64 = k.out_type == MIGRID_F64;
+    switch (k.stage) {
+    case kStageBf16:
+        if (out64)
+            return in64 ? run_modes<Bf16x2, float, RecF32, double, double>(k, L, s)
+                        : run_modes<Bf16x2, float, RecF32, float, double>(k, L, s);
+        return in64 ? run_modes<Bf16x2, float, RecF32, double, float>(k, L, s)
+                    : run_modes<Bf16x2, float, RecF32, float, float>(k, L, s);
+    case kStageF32:
+        if (k.acc64)  // float products summed in double (native <double2,float2>)
+            return run_modes<float2, double, RecF32, float, double>(k, L, s);
+        if (out64)
+            return in64 ? run_modes<float2, float, RecF32, double, double>(k, L, s)
+                        : run_modes<float2, float, RecF32, float, double>(k, L, s);
+        return in64 ? run_modes<float2, float, RecF32, double, float>(k, L, s)
+                    : run_modes<float2, float, RecF32, float, float>(k, L, s);
+    default:
+        return run_modes<double2, double, RecF64, double, double>(k, L, s);
+    }
+}
 
 TileCfg pick_tile(int S, int tile_opt)
 {
@@ -435,207 +547,170 @@ TileCfg pick_tile(int S, int tile_opt)
     return c;
 }
 
-struct Launch {
-    const void* rec;
-    const int32_t* offs;
-    const Item* items;
-    QueueState* qs;
-    const uint8_t* split;
-    const void* gcf;
-    void* grid;
-    void* grid_lo;
-    TileGeom g;
-    double sigma, beta;
-    size_t lds;
-    int blocks;
-};
-
-template <int RX, int RY, int WX, int WY, typename G, typename TA, typename R, typename TI,
-          typename TO, bool PLANAR, int GM>
-int run_owner(const Launch& L, hipStream_t s)
+int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
 {
-    auto kern = k_owner<RX, RY, WX, WY, G, TA, R, TI, TO, PLANAR, GM>;
-    MG_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)L.lds));
-    int per_cu = 0;
-    MG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * WX * WY, L.lds));
-    if (per_cu < 1) return fail(MIGRID_ENOTSUP, "tiled: kernel does not fit (lds=%zu)", L.lds);
-    int dev = 0, ncu = 0;
-    MG_HIP(hipGetDevice(&dev));
-    MG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const int blocks = ncu * per_cu;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * WX * WY), L.lds, s, (const R*)L.rec, L.offs,
-                       L.items, L.qs, L.split, (const TI*)L.gcf, (TO*)L.grid, (TO*)L.grid_lo, L.g,
-                       L.sigma, L.beta);
+    if (n < 1) return fail(MIGRID_EINVAL, "tiled plan with %d jobs", n);
+    const bool f32arith = o.arith == MIGRID_ARITH_F32 || o.arith == MIGRID_ARITH_F32_BF16GCF;
+    if (o.strategy != MIGRID_TILED && o.arith != MIGRID_ARITH_NATIVE)
+        return fail(MIGRID_ENOTSUP, "gather/window run native arithmetic only");
+    if (o.tile != 0 && o.tile != 16 && o.tile != 32 && o.tile != 64)
+        return fail(MIGRID_EINVAL, "tile=%d (0, 16, 32 or 64)", o.tile);
+    p = TiledPlan();
+    p.jobs.resize(n);
+    size_t off = 0;
+    int64_t item_cap = 0;
+    for (int j = 0; j < n; j++) {
+        const GridArgs& x = a[j];
+        if (x.S > 64) return fail(MIGRID_ENOTSUP, "tiled strategies support S <= 64 (S=%d)", x.S);
+        if (x.N + x.S - 2 >= 65536) return fail(MIGRID_ENOTSUP, "tiled: N + S - 2 must be < 65536");
+        if (x.debug && (f32arith || x.out_type != MIGRID_F64))
+            return fail(MIGRID_ENOTSUP, "coverage mode needs native arithmetic into a double grid");
+        if (x.grid_lo && x.out_type != MIGRID_F32)
+            return fail(MIGRID_ENOTSUP, "a compensation grid needs a float grid");
+        KernelKey k;
+        k.stage = o.arith == MIGRID_ARITH_F32_BF16GCF ? kStageBf16
+                  : (f32arith || x.in_type == MIGRID_F32) ? kStageF32
+                                                         : kStageF64;
+        k.acc64 = !f32arith && x.in_type == MIGRID_F32 && x.out_type == MIGRID_F64;
+        k.in_type = x.in_type;
+        k.out_type = x.out_type;
+        k.planar = x.planar;
+        k.gm = (x.debug || x.gcf_mode == MIGRID_GCF_COMPAT) ? 1
+               : x.gcf_mode == MIGRID_GCF_ANALYTIC        ? 2
+                                                          : 0;
+        const TileCfg tc = pick_tile(x.S, o.tile);
+        k.T = tc.T;
+        if (j == 0)
+            p.key = k;
+        else if (!(k == p.key))
+            return fail(MIGRID_ENOTSUP, "batched jobs must share types, layout, GCF mode and tile");
+        TiledJob& t = p.jobs[j];
+        t.a = x;
+        t.bg = bin_geom(x.N, x.S, tc.H);
+        if (t.bg.nkeys >= ((int64_t)1 << 31)) return fail(MIGRID_ENOTSUP, "tiled: too many bins");
+        t.g.N = x.N;
+        t.g.S = x.S;
+        t.g.T = tc.T;
+        t.g.H = tc.H;
+        t.g.nb = t.bg.nb;
+        t.g.ntx = (x.N + tc.T - 1) / tc.T;
+        t.rec64 = k.stage == kStageF64;
+        t.vt = kValPlain;
+        if (x.debug)
+            t.vt = kValDebug;
+        else if (x.gcf_mode == MIGRID_GCF_COMPAT)
+            t.vt = o.strategy == MIGRID_TILED ? kValCompatS : kValCompatI;
+        const int64_t ntiles = (int64_t)t.g.ntx * t.g.ntx;
+        t.off_rec = off;
+        off += align256((size_t)x.npts * (t.rec64 ? sizeof(RecF64) : sizeof(RecF32)));
+        t.off_perm = off;
+        off += align256((size_t)(x.npts > 0 ? x.npts : 1) * 4);
+        t.off_offs = off;
+        off += align256(((size_t)t.bg.nkeys + 1) * 4);
+        t.off_split = off;
+        off += align256((size_t)ntiles);
+        item_cap += ntiles * kSubKernels;
+        p.bin_bytes = std::max(p.bin_bytes, bin_scratch_bytes(x.npts, t.bg));
+        p.smax = std::max(p.smax, x.S);
+    }
+    p.off_desc = off;
+    off += align256(sizeof(JobDesc) * n);
+    p.off_items = off;
+    off += align256((size_t)item_cap * sizeof(Item));
+    p.off_qs = off;
+    off += align256(sizeof(QueueState));
+    p.arena_bytes = off;
+    p.item_cap = item_cap;
+    p.target = o.strategy == MIGRID_TILED ? (o.vis_per_wave > 0 ? o.vis_per_wave : 4096) : 0;
+    const size_t gsz = p.key.stage == kStageBf16 ? 4 : p.key.stage == kStageF32 ? 8 : 16;
+    p.lds = (size_t)(p.smax + 2 * kPad) * (p.smax + 2 * kPad) * gsz;
+    return MIGRID_OK;
+}
+
+void tiled_descs(const TiledPlan& p, char* arena, std::vector<JobDesc>& d)
+{
+    d.resize(p.jobs.size());
+    for (size_t j = 0; j < p.jobs.size(); j++) {
+        const TiledJob& t = p.jobs[j];
+        d[j].g = t.g;
+        d[j].rec = arena + t.off_rec;
+        d[j].offs = (const int32_t*)(arena + t.off_offs);
+        d[j].split = (const uint8_t*)(arena + t.off_split);
+        d[j].gcf = t.a.gcf;
+        d[j].grid = t.a.grid;
+        d[j].grid_lo = t.a.grid_lo;
+        d[j].sigma = t.a.sigma;
+        d[j].beta = t.a.beta;
+    }
+}
+
+int tiled_write_descs(const TiledPlan& p, char* arena, const std::vector<JobDesc>& d,
+                      hipStream_t s)
+{
+    JobDesc* dst = (JobDesc*)(arena + p.off_desc);
+    for (size_t j = 0; j < d.size(); j++)
+        hipLaunchKernelGGL(k_set_desc, dim3(1), dim3(1), 0, s, dst + j, d[j]);
     MG_HIP(hipGetLastError());
     return MIGRID_OK;
 }
 
-// Shape dispatch: T = 64 uses 2x2 waves of 4x4 blocks, T = 32 2x2 waves of 2x2.
-template <typename G, typename TA, typename R, typename TI, typename TO, bool PLANAR, int GM>
-int run_shape(const Launch& L, hipStream_t s)
+int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_t s)
 {
-    if (L.g.T == 64) return run_owner<4, 4, 2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);
-    if (L.g.T == 32) return run_owner<2, 2, 2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);
-    return fail(MIGRID_ENOTSUP, "tiled: tile edge %d (32 or 64 supported)", L.g.T);
-}
-
-template <typename G, typename TA, typename R, typename TI, typename TO>
-int run_modes(const Launch& L, bool planar, int gm, hipStream_t s)
-{
-    if (planar) {
-        if (gm == 0) return run_shape<G, TA, R, TI, TO, true, 0>(L, s);
-        if (gm == 1) return run_shape<G, TA, R, TI, TO, true, 1>(L, s);
-        return run_shape<G, TA, R, TI, TO, true, 2>(L, s);
+    QueueState* qs = (QueueState*)(arena + p.off_qs);
+    Item* items = (Item*)(arena + p.off_items);
+    MG_TRY(fill_u32(qs, 0u, sizeof(QueueState) / 4, s));
+    for (size_t j = 0; j < p.jobs.size(); j++) {
+        const TiledJob& t = p.jobs[j];
+        const GridArgs& a = t.a;
+        int32_t* perm = (int32_t*)(arena + t.off_perm);
+        int32_t* offs = (int32_t*)(arena + t.off_offs);
+        MG_TRY(bin_sort(t.bg, a.in_type, a.uv, a.npts, bin_scratch, perm, offs, s));
+        const int32_t* count = offs + t.bg.nkeys;
+        if (a.npts > 0) {
+            if (a.in_type == MIGRID_F64) {
+                if (t.rec64)
+                    launch_pack<double, RecF64>(a, t.vt, perm, count, (RecF64*)(arena + t.off_rec), s);
+                else
+                    launch_pack<double, RecF32>(a, t.vt, perm, count, (RecF32*)(arena + t.off_rec), s);
+            } else {
+                launch_pack<float, RecF32>(a, t.vt, perm, count, (RecF32*)(arena + t.off_rec), s);
+            }
+            MG_HIP(hipGetLastError());
+        }
+        const int64_t ntiles = (int64_t)t.g.ntx * t.g.ntx;
+        hipLaunchKernelGGL(k_items, dim3((unsigned)((ntiles + 3) / 4)), dim3(256), 0, s, offs, t.g,
+                           p.target, (int)j, items, qs, (uint8_t*)(arena + t.off_split));
+        MG_HIP(hipGetLastError());
     }
-    if (gm == 0) return run_shape<G, TA, R, TI, TO, false, 0>(L, s);
-    if (gm == 1) return run_shape<G, TA, R, TI, TO, false, 1>(L, s);
-    return run_shape<G, TA, R, TI, TO, false, 2>(L, s);
+    return MIGRID_OK;
 }
 
-template <typename TI, typename R>
-void launch_pack(const GridArgs& a, int vt, const int32_t* perm, const int32_t* count, R* rec,
-                 hipStream_t s)
+int tiled_execute(const TiledPlan& p, char* arena, hipStream_t s)
 {
-    const int blocks = (int)std::min<int64_t>(4096, (a.npts + 255) / 256 + 1);
-    switch (vt) {
-    case kValCompatS:
-        hipLaunchKernelGGL((k_pack<TI, R, kValCompatS>), dim3(blocks), dim3(256), 0, s,
-                           (const TI*)a.uv, (const TI*)a.vals, perm, count, a.S, (const TI*)a.gcf, rec);
-        break;
-    case kValCompatI:
-        hipLaunchKernelGGL((k_pack<TI, R, kValCompatI>), dim3(blocks), dim3(256), 0, s,
-                           (const TI*)a.uv, (const TI*)a.vals, perm, count, a.S, (const TI*)a.gcf, rec);
-        break;
-    case kValDebug:
-        hipLaunchKernelGGL((k_pack<TI, R, kValDebug>), dim3(blocks), dim3(256), 0, s,
-                           (const TI*)a.uv, (const TI*)a.vals, perm, count, a.S, (const TI*)a.gcf, rec);
-        break;
-    default:
-        hipLaunchKernelGGL((k_pack<TI, R, kValPlain>), dim3(blocks), dim3(256), 0, s,
-                           (const TI*)a.uv, (const TI*)a.vals, perm, count, a.S, (const TI*)a.gcf, rec);
-    }
+    OwnerLaunch L;
+    L.jobs = (const JobDesc*)(arena + p.off_desc);
+    L.items = (const Item*)(arena + p.off_items);
+    L.qs = (QueueState*)(arena + p.off_qs);
+    L.smax = p.smax;
+    L.lds = p.lds;
+    return dispatch_owner(p.key, L, s);
 }
-
-}  // namespace
 
 int launch_tiled(migrid_ctx* ctx, const GridArgs& a, const migrid_opts& o, hipStream_t s)
 {
     if (!ctx) return fail(MIGRID_EINVAL, "tiled strategies need a context");
-    if (a.S > 64) return fail(MIGRID_ENOTSUP, "tiled strategies support S <= 64 (S=%d)", a.S);
-    if (a.N + a.S - 2 >= 65536) return fail(MIGRID_ENOTSUP, "tiled: N + S - 2 must be < 65536");
-    const bool f32arith = o.arith == MIGRID_ARITH_F32 || o.arith == MIGRID_ARITH_F32_BF16GCF;
-    const bool bf16 = o.arith == MIGRID_ARITH_F32_BF16GCF;
-    if (o.strategy != MIGRID_TILED && o.arith != MIGRID_ARITH_NATIVE)
-        return fail(MIGRID_ENOTSUP, "gather/window run native arithmetic only");
-    if (a.debug && (f32arith || a.out_type != MIGRID_F64))
-        return fail(MIGRID_ENOTSUP, "coverage mode needs native arithmetic into a double grid");
-    if (a.grid_lo && !(a.out_type == MIGRID_F32))
-        return fail(MIGRID_ENOTSUP, "a compensation grid needs a float grid");
+    TiledPlan p;
+    MG_TRY(tiled_plan(&a, 1, o, p));
     if (a.npts == 0) return MIGRID_OK;
-
-    const TileCfg tc = pick_tile(a.S, o.tile);
-    const BinGeom bg = bin_geom(a.N, a.S, tc.H);
-    if (bg.nkeys >= ((int64_t)1 << 31)) return fail(MIGRID_ENOTSUP, "tiled: too many bins");
-    TileGeom g;
-    g.N = a.N;
-    g.S = a.S;
-    g.T = tc.T;
-    g.H = tc.H;
-    g.nb = bg.nb;
-    g.ntx = (a.N + tc.T - 1) / tc.T;
-    const int64_t ntiles = (int64_t)g.ntx * g.ntx;
-
-    // records: 16 B (float values) or 32 B (double values)
-    const bool rec64 = !f32arith && a.in_type == MIGRID_F64;
-    const size_t rec_bytes = (size_t)a.npts * (rec64 ? sizeof(RecF64) : sizeof(RecF32));
-    const size_t perm_bytes = ((size_t)a.npts * 4 + 255) & ~(size_t)255;
-    const size_t offs_bytes = (((size_t)bg.nkeys + 1) * 4 + 255) & ~(size_t)255;
-    const size_t item_cap = (size_t)ntiles * kSubKernels;
-    const size_t items_bytes = (item_cap * sizeof(Item) + 255) & ~(size_t)255;
-    const size_t split_bytes = ((size_t)ntiles + 255) & ~(size_t)255;
-    MG_TRY(ctx->bins.ensure(bin_scratch_bytes(a.npts, bg)));
-    MG_TRY(ctx->records.ensure(rec_bytes + perm_bytes + offs_bytes));
-    MG_TRY(ctx->queue.ensure(items_bytes + split_bytes + sizeof(QueueState)));
-    char* rp = (char*)ctx->records.p;
-    void* rec = rp;
-    int32_t* perm = (int32_t*)(rp + ((rec_bytes + 255) & ~(size_t)255));
-    int32_t* offs = (int32_t*)((char*)perm + perm_bytes);
-    // (records region sized with slack for the alignment above)
-    Item* items = (Item*)ctx->queue.p;
-    uint8_t* split = (uint8_t*)ctx->queue.p + items_bytes;
-    QueueState* qs = (QueueState*)((char*)ctx->queue.p + items_bytes + split_bytes);
-
-    MG_TRY(bin_sort(bg, a.in_type, a.uv, a.npts, ctx->bins.p, perm, offs, s));
-
-    int vt = kValPlain;
-    if (a.debug)
-        vt = kValDebug;
-    else if (a.gcf_mode == MIGRID_GCF_COMPAT)
-        vt = o.strategy == MIGRID_TILED ? kValCompatS : kValCompatI;
-    const int32_t* count = offs + bg.nkeys;
-    if (a.in_type == MIGRID_F64) {
-        if (rec64)
-            launch_pack<double, RecF64>(a, vt, perm, count, (RecF64*)rec, s);
-        else
-            launch_pack<double, RecF32>(a, vt, perm, count, (RecF32*)rec, s);
-    } else {
-        launch_pack<float, RecF32>(a, vt, perm, count, (RecF32*)rec, s);
-    }
-    MG_HIP(hipGetLastError());
-
-    MG_HIP(hipMemsetAsync(qs, 0, sizeof(QueueState), s));
-    const int64_t target = o.strategy == MIGRID_TILED ? (o.vis_per_wave > 0 ? o.vis_per_wave : 4096) : 0;
-    hipLaunchKernelGGL(k_items, dim3((unsigned)((ntiles + 3) / 4)), dim3(256), 0, s, offs, g, target,
-                       items, qs, split);
-    MG_HIP(hipGetLastError());
-
-    Launch L;
-    L.rec = rec;
-    L.offs = offs;
-    L.items = items;
-    L.qs = qs;
-    L.split = split;
-    L.gcf = a.gcf;
-    L.grid = a.grid;
-    L.grid_lo = a.grid_lo;
-    L.g = g;
-    L.sigma = a.sigma;
-    L.beta = a.beta;
-    const int gm = (a.debug || a.gcf_mode == MIGRID_GCF_COMPAT) ? 1
-                   : a.gcf_mode == MIGRID_GCF_ANALYTIC     ? 2
-                                                           : 0;
-    const size_t cells = (size_t)(a.S + 2 * kPad) * (a.S + 2 * kPad);
+    MG_TRY(ctx->bins.ensure(p.bin_bytes));
+    MG_TRY(ctx->records.ensure(p.arena_bytes));
+    char* arena = (char*)ctx->records.p;
+    std::vector<JobDesc> d;
+    tiled_descs(p, arena, d);
+    MG_TRY(tiled_write_descs(p, arena, d, s));
+    MG_TRY(tiled_prepare(p, arena, ctx->bins.p, s));
     MG_TRY(ctx->prof.begin(s));
-    int rc;
-    const bool in64 = a.in_type == MIGRID_F64, out64 = a.out_type == MIGRID_F64;
-    if (bf16) {
-        L.lds = cells * sizeof(Bf16x2);
-        if (out64)
-            rc = in64 ? run_modes<Bf16x2, float, RecF32, double, double>(L, a.planar, gm, s)
-                      : run_modes<Bf16x2, float, RecF32, float, double>(L, a.planar, gm, s);
-        else
-            rc = in64 ? run_modes<Bf16x2, float, RecF32, double, float>(L, a.planar, gm, s)
-                      : run_modes<Bf16x2, float, RecF32, float, float>(L, a.planar, gm, s);
-    } else if (f32arith) {
-        L.lds = cells * sizeof(float2);
-        if (out64)
-            rc = in64 ? run_modes<float2, float, RecF32, double, double>(L, a.planar, gm, s)
-                      : run_modes<float2, float, RecF32, float, double>(L, a.planar, gm, s);
-        else
-            rc = in64 ? run_modes<float2, float, RecF32, double, float>(L, a.planar, gm, s)
-                      : run_modes<float2, float, RecF32, float, float>(L, a.planar, gm, s);
-    } else if (a.in_type == MIGRID_F64) {  // zz
-        L.lds = cells * sizeof(double2);
-        rc = run_modes<double2, double, RecF64, double, double>(L, a.planar, gm, s);
-    } else if (a.out_type == MIGRID_F64) {  // zc: float products, double sums
-        L.lds = cells * sizeof(float2);
-        rc = run_modes<float2, double, RecF32, float, double>(L, a.planar, gm, s);
-    } else {  // cc
-        L.lds = cells * sizeof(float2);
-        rc = run_modes<float2, float, RecF32, float, float>(L, a.planar, gm, s);
-    }
-    if (rc != MIGRID_OK) return rc;
+    MG_TRY(tiled_execute(p, arena, s));
     MG_TRY(ctx->prof.end(s));
     return MIGRID_OK;
 }

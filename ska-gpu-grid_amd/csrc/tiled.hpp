@@ -4,6 +4,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
+
+#include "bin.hpp"
+#include "internal.hpp"
 
 namespace migrid {
 
@@ -21,11 +25,12 @@ struct alignas(16) RecF64 {
     double vr, vi;
 };
 
-// One unit of work: a tile and an inclusive sub-kernel range.  split = the
-// tile has more than one item, so its flush must be atomic.
+// One unit of work: a job, a tile of it and an inclusive sub-kernel range.
 struct Item {
     int32_t tile;
     int16_t klo, khi;
+    int32_t job;
+    int32_t pad;
 };
 
 struct QueueState {
@@ -41,5 +46,85 @@ struct TileGeom {
     int nb;   // bins per axis
     int ntx;  // tiles per axis
 };
+
+// Everything the owner kernel needs about one job (device-resident table,
+// indexed by Item::job).  split[tile] != 0: the tile has more than one item,
+// so its flush must be atomic.
+struct JobDesc {
+    TileGeom g;
+    const void* rec;
+    const int32_t* offs;
+    const uint8_t* split;
+    const void* gcf;
+    void* grid;
+    void* grid_lo;
+    double sigma, beta;
+};
+
+// ---- host-side plan ---------------------------------------------------------
+constexpr int kStageF64 = 0, kStageF32 = 1, kStageBf16 = 2;
+
+// Everything that selects the owner-kernel instantiation; one launch serves
+// jobs with equal keys.
+struct KernelKey {
+    int stage = kStageF64;  // staged coefficient type
+    bool acc64 = false;     // float products summed in double
+    int in_type = 0, out_type = 0;
+    bool planar = false;
+    int gm = 0;  // 0 table, 1 constant tap (COMPAT/coverage), 2 analytic
+    int T = 32;
+    bool operator==(const KernelKey& o) const
+    {
+        return stage == o.stage && acc64 == o.acc64 && in_type == o.in_type &&
+               out_type == o.out_type && planar == o.planar && gm == o.gm && T == o.T;
+    }
+};
+
+struct TileCfg {
+    int T;
+    int H;
+};
+TileCfg pick_tile(int S, int tile_opt);
+
+struct TiledJob {
+    GridArgs a;
+    BinGeom bg;
+    TileGeom g;
+    bool rec64 = false;
+    int vt = 0;
+    size_t off_rec = 0, off_perm = 0, off_offs = 0, off_split = 0;  // in the arena
+};
+
+// Device layout of one owner launch over n jobs: per-job records, bin-sort
+// output and split flags, then the job table, the item queue and its state.
+struct TiledPlan {
+    KernelKey key;
+    std::vector<TiledJob> jobs;
+    size_t off_desc = 0, off_items = 0, off_qs = 0, arena_bytes = 0;
+    size_t bin_bytes = 0;  // bin-sort scratch (reused job after job)
+    int64_t item_cap = 0;
+    int64_t target = 0;  // visibilities per work item (0: one item per tile)
+    int smax = 0;
+    size_t lds = 0;
+};
+
+struct OwnerLaunch {
+    const JobDesc* jobs;
+    const Item* items;
+    QueueState* qs;
+    int smax;
+    size_t lds;
+};
+
+int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p);
+void tiled_descs(const TiledPlan& p, char* arena, std::vector<JobDesc>& d);
+// Writes the job table into the arena with kernels (graph-safe).
+int tiled_write_descs(const TiledPlan& p, char* arena, const std::vector<JobDesc>& d,
+                      hipStream_t s);
+// Queue reset, then per job: bin sort, record packing, work items.
+int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_t s);
+// The persistent owner kernel over all queued items.
+int tiled_execute(const TiledPlan& p, char* arena, hipStream_t s);
+int dispatch_owner(const KernelKey& k, const OwnerLaunch& L, hipStream_t s);
 
 }  // namespace migrid
