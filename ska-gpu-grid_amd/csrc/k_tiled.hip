@@ -6,21 +6,24 @@
 // Gridding is a weighted scatter, not a dense contraction, so MFMA is not
 // used; the kernel is built for wave64 VALU + LDS:
 //
-//  * The grid is cut into T x T tiles.  A workgroup of WX x WY waves owns one
-//    tile; wave (wx, wy) owns an (8*RX) x (8*RY) block of it, and lane
-//    (lx, ly) = (lane & 7, lane >> 3) owns the RX x RY pixels
+//  * The grid is cut into T x T tiles (T = 16, 32, 64).  A workgroup of
+//    (T/16)^2 waves owns one tile; each wave owns a 16 x 16 block of it, and
+//    lane (lx, ly) = (lane & 7, lane >> 3) owns the 2 x 2 pixels
 //    (Xw + 8j + lx, Yw + 8i + ly), accumulated in registers.
 //  * Visibilities are bin-sorted by (k, bin_y, bin_x) over q = floor(uv)+S/2
-//    (k_bin.hip).  Pixel x receives visibilities with q_x in [x, x+S-1], so a
-//    tile's visibilities for one k are a few contiguous CSR ranges.
-//  * For each k the workgroup stages sub-kernel k into LDS with a zero border
-//    of 7 on every side.  For a visibility, an 8 x 8 register block is either
-//    untouched (skipped on a wave-uniform test) or reads its 64 coefficients
-//    with one ds_read per lane at an immediate offset; lanes outside the
-//    S x S window read the zero border, so no per-lane masks are needed.
-//  * A visibility record (q, value) is wave-uniform and comes through scalar
-//    loads; the complex multiply-accumulate is 4 FMAs per tap with the value
-//    in SGPRs.
+//    in bins of 16 (k_bin.hip).  Pixel x receives visibilities with q_x in
+//    [x, x+S-1], so a wave's visibilities for one k are a few contiguous CSR
+//    ranges, and every one of them reaches the wave's block.
+//  * For each k the workgroup stages sub-kernel k into LDS inside a zero
+//    border wide enough for any listed visibility: each visibility costs a
+//    lane four ds_reads at immediate offsets and 16 FMAs, with no per-lane
+//    masks and no branches (taps outside the window read zeros).  A row pitch
+//    of 8 mod 32 elements keeps the lane block's rows in distinct banks.  A k
+//    with few visibilities for the workgroup skips staging and reads the
+//    table from HBM with per-lane window masks.
+//  * Visibility records (q, value) are wave-uniform and come through scalar
+//    loads (constant address space); the complex multiply-accumulate is 4
+//    FMAs per tap with the value in SGPRs.
 //  * Work items are (tile, k-range) pairs sized by visibility count, taken
 //    from a device queue by persistent workgroups.  A tile that is one item
 //    is flushed with a plain read-add-write; a split tile with hardware
@@ -41,7 +44,6 @@
 namespace migrid {
 namespace {
 
-constexpr int kPad = 7;  // zero border of a staged sub-kernel
 
 // ---- staged coefficient types ------------------------------------------------
 struct Bf16x2 {
@@ -262,11 +264,51 @@ __device__ __forceinline__ void flush_px(TO* grid, TO* lo, int64_t idx_re, int64
 }
 
 // ---- the owner kernel -----------------------------------------------------------
-// G: staged coefficient type; TA: accumulator type; R: record type;
-// TI: type of the gcf table in HBM; TO: grid type.
-// GM: 0 table, 1 ones (COMPAT/DEBUG records carry the tap value), 2 analytic.
-template <int RX, int RY, int WX, int WY, typename G, typename TA, typename R, typename TI,
-          typename TO, bool PLANAR, int GM>
+// Zero border of a staged sub-kernel: a wave covers a 16 x 16 tile and its
+// visibility bins are 16 pixels wide, so a listed visibility can sit up to
+// 16*ceil((S+15)/16) - 1 pixels past the tile's origin; the border makes every
+// such read land in the staged kernel or in zeros.
+__host__ __device__ constexpr int stage_border(int S) { return 16 * ((S + 15 + 15) / 16) - S; }
+__host__ __device__ constexpr int stage_rows(int S) { return S + 2 * stage_border(S); }
+// Row pitch = 8 (mod 32) elements: the 8 rows of an 8 x 8 lane block fall in
+// different LDS banks for 4-, 8- and 16-byte elements.
+__host__ __device__ constexpr int stage_pitch(int S)
+{
+    return stage_rows(S) + ((8 - stage_rows(S) % 32) + 32) % 32;
+}
+constexpr int kStageMin = 32;  // fewer visibilities for a k: read the table from HBM
+
+template <typename R>
+using ConstPtr = const __attribute__((address_space(4))) R*;  // scalar-loadable
+template <typename R>
+using GlobalPtr = const __attribute__((address_space(1))) R*;
+
+// acc[i][j] += v * g[i][j] for the 2 x 2 pixels of a lane: 4 FMAs per tap, or
+// float products summed in double for the <double2,float2> pair.
+template <typename P, typename TA>
+__device__ __forceinline__ void mac4(Cplx<TA> (&acc)[2][2], P vr, P vi, const P (&gr)[2][2],
+                                     const P (&gi)[2][2])
+{
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            if (sizeof(P) == sizeof(TA)) {
+                acc[i][j].re = fma((TA)vr, (TA)gr[i][j], acc[i][j].re);
+                acc[i][j].re = fma(-(TA)vi, (TA)gi[i][j], acc[i][j].re);
+                acc[i][j].im = fma((TA)vr, (TA)gi[i][j], acc[i][j].im);
+                acc[i][j].im = fma((TA)vi, (TA)gr[i][j], acc[i][j].im);
+            } else {
+                acc[i][j].re += (TA)(vr * gr[i][j] - vi * gi[i][j]);
+                acc[i][j].im += (TA)(vr * gi[i][j] + gr[i][j] * vi);
+            }
+        }
+}
+
+// WX x WY waves own a (16 WX) x (16 WY) tile; lane (lx, ly) of wave (wx, wy)
+// owns pixels (Xw + lx + 8j, Yw + ly + 8i), i, j in {0, 1}.
+template <int WX, int WY, typename G, typename TA, typename R, typename TI, typename TO,
+          bool PLANAR, int GM>
 __global__ __launch_bounds__(64 * WX * WY) void k_owner(const JobDesc* __restrict__ jobs,
                                                         const Item* __restrict__ items,
                                                         QueueState* __restrict__ qs, int smax)
@@ -280,13 +322,11 @@ __global__ __launch_bounds__(64 * WX * WY) void k_owner(const JobDesc* __restric
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int w = tid >> 6;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     const int wx = w % WX, wy = w / WX;
     const int lx = lane & 7, ly = lane >> 3;
 
-    // zero the whole staging buffer once: the border stays zero as long as
-    // the support (hence the staged layout) does not change between items
-    for (int e = tid; e < (smax + 2 * kPad) * (smax + 2 * kPad); e += NT)
+    for (int e = tid; e < stage_rows(smax) * stage_pitch(smax); e += NT)
         sg[e] = Stage<G>::make(0.0, 0.0);
     int staged_s = -1;  // support of the layout now in LDS
 
@@ -301,60 +341,63 @@ __global__ __launch_bounds__(64 * WX * WY) void k_owner(const JobDesc* __restric
         const int khi = __builtin_amdgcn_readfirstlane(it.khi);
         const int job = __builtin_amdgcn_readfirstlane(it.job);
         const TileGeom g = jobs[job].g;
-        const R* __restrict__ rec = (const R*)jobs[job].rec;
-        const int32_t* __restrict__ offs = jobs[job].offs;
+        ConstPtr<R> rec = (ConstPtr<R>)(uintptr_t)jobs[job].rec;
+        ConstPtr<int32_t> offs = (ConstPtr<int32_t>)(uintptr_t)jobs[job].offs;
         const TI* __restrict__ gcf = (const TI*)jobs[job].gcf;
         TO* __restrict__ grid = (TO*)jobs[job].grid;
         TO* __restrict__ grid_lo = (TO*)jobs[job].grid_lo;
         const double sigma = jobs[job].sigma, beta = jobs[job].beta;
         const bool atomic = jobs[job].split[tile] != 0;
         const int S = g.S;
-        const int pitch = S + 2 * kPad;
-        const int lane_off = (ly + kPad) * pitch + lx + kPad;
+        const int B = stage_border(S);
+        const int pitch = stage_pitch(S);
         const int64_t NN = (int64_t)g.N * g.N;
         if (S != staged_s) {
-            // a new layout: clear the old interior (border positions of the
-            // new layout may have been interior before), then, for the
-            // constant-tap form, write the all-ones kernel once
+            // a new layout: clear the old interior, then (constant-tap form)
+            // write the all-ones kernel once
             __syncthreads();
             if (staged_s > 0) {
-                const int op = staged_s + 2 * kPad;
+                const int ob = stage_border(staged_s), op = stage_pitch(staged_s);
                 for (int e = tid; e < staged_s * staged_s; e += NT)
-                    sg[(e / staged_s + kPad) * op + e % staged_s + kPad] = Stage<G>::make(0.0, 0.0);
+                    sg[(e / staged_s + ob) * op + e % staged_s + ob] = Stage<G>::make(0.0, 0.0);
                 __syncthreads();
             }
             if (GM == 1)
                 for (int e = tid; e < S * S; e += NT)
-                    sg[(e / S + kPad) * pitch + e % S + kPad] = Stage<G>::make(1.0, 0.0);
+                    sg[(e / S + B) * pitch + e % S + B] = Stage<G>::make(1.0, 0.0);
             staged_s = S;
         }
         __syncthreads();
         const int tx = tile % g.ntx, ty = tile / g.ntx;
         const int X0 = tx * g.T, Y0 = ty * g.T;
-        const int Xw = X0 + wx * 8 * RX, Yw = Y0 + wy * 8 * RY;
-        const int bxl = X0 / g.H, byl = Y0 / g.H;
-        const int bxh = min(g.nb - 1, (X0 + g.T + S - 2) / g.H);
-        const int byh = min(g.nb - 1, (Y0 + g.T + S - 2) / g.H);
+        const int Xw = X0 + 16 * wx, Yw = Y0 + 16 * wy;
+        // this wave's bins (q in [Xw, Xw + S + 14]) and the workgroup's
+        const int bxl = Xw / g.H, bxh = min(g.nb - 1, (Xw + S + 14) / g.H);
+        const int byl = Yw / g.H, byh = min(g.nb - 1, (Yw + S + 14) / g.H);
+        const int Bxl = X0 / g.H, Bxh = min(g.nb - 1, (X0 + g.T + S - 2) / g.H);
+        const int Byl = Y0 / g.H, Byh = min(g.nb - 1, (Y0 + g.T + S - 2) / g.H);
+        const int lane_off = ly * pitch + lx;
 
-        Cplx<TA> acc[RY][RX];
+        Cplx<TA> acc[2][2];
 #pragma unroll
-        for (int i = 0; i < RY; i++)
+        for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int j = 0; j < RX; j++) acc[i][j].re = acc[i][j].im = (TA)0;
+            for (int j = 0; j < 2; j++) acc[i][j].re = acc[i][j].im = (TA)0;
 
         for (int k = klo; k <= khi; k++) {
-            int64_t nk = 0;
-            for (int by = byl; by <= byh; by++) {
+            int nk = 0;
+            for (int by = Byl; by <= Byh; by++) {
                 const int64_t key = ((int64_t)k * g.nb + by) * g.nb;
-                nk += offs[key + bxh + 1] - offs[key + bxl];
+                nk += offs[key + Bxh + 1] - offs[key + Bxl];
             }
             if (nk == 0) continue;
-            if (GM != 1) {
+            const bool staged = GM != 0 || nk >= kStageMin;
+            if (staged && GM != 1) {
                 __syncthreads();  // previous sub-kernel no longer read
                 if (GM == 0) {
                     const TI* gk = gcf + 2 * (int64_t)k * S * S;
                     for (int e = tid; e < S * S; e += NT)
-                        sg[(e / S + kPad) * pitch + e % S + kPad] =
+                        sg[(e / S + B) * pitch + e % S + B] =
                             Stage<G>::make((double)gk[2 * e], (double)gk[2 * e + 1]);
                 } else {
                     // separable analytic GCF: G[r][c] = f(dy_r) * f(dx_c),
@@ -374,59 +417,65 @@ __global__ __launch_bounds__(64 * WX * WY) void k_owner(const JobDesc* __restric
                     for (int e = tid; e < S * S; e += NT) {
                         const int r = e / S, c = e % S;
                         const Cplx<double> a = s_fac[1][r], b = s_fac[0][c];
-                        sg[(r + kPad) * pitch + c + kPad] =
+                        sg[(r + B) * pitch + c + B] =
                             Stage<G>::make(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
                     }
                 }
                 __syncthreads();
             }
+            GlobalPtr<TI> gk = (GlobalPtr<TI>)(uintptr_t)(gcf + 2 * (int64_t)k * S * S);
             for (int by = byl; by <= byh; by++) {
                 const int64_t key = ((int64_t)k * g.nb + by) * g.nb;
-                const int r0 = __builtin_amdgcn_readfirstlane(offs[key + bxl]);
-                const int r1 = __builtin_amdgcn_readfirstlane(offs[key + bxh + 1]);
-                for (int r = r0; r < r1; r++) {
-                    const R vrec = rec[r];
-                    const int qx = (int)(vrec.q & 0xffffu), qy = (int)(vrec.q >> 16);
-                    // column/row of the staged kernel read by lane (0,0), block (0,0)
-                    const int c0 = Xw - qx + S - 1;
-                    const int rr0 = Yw - qy + S - 1;
-                    // touched blocks: c0 + 8j in [-7, S-1]
-                    const int jlo = max(0, (c0 < -kPad) ? (-kPad - c0 + 7) >> 3 : 0);
-                    const int jhi = min(RX - 1, (S - 1 - c0) >= 0 ? (S - 1 - c0) >> 3 : -1);
-                    const int ilo = max(0, (rr0 < -kPad) ? (-kPad - rr0 + 7) >> 3 : 0);
-                    const int ihi = min(RY - 1, (S - 1 - rr0) >= 0 ? (S - 1 - rr0) >> 3 : -1);
-                    if (jlo > jhi || ilo > ihi) continue;
-                    const P vr = (P)vrec.vr, vi = (P)vrec.vi;
-                    const G* base = sg + lane_off + rr0 * pitch + c0;
+                const int r0 = offs[key + bxl];
+                const int r1 = offs[key + bxh + 1];
+                if (staged) {
+                    // per lane: (row, column) of block (0,0) for q = (Xw, Yw),
+                    // shifted per visibility by a wave-uniform offset
+                    const G* lb = sg + lane_off + (S - 1 + B) * pitch + (S - 1 + B);
+#pragma unroll 4
+                    for (int r = r0; r < r1; r++) {
+                        const uint32_t q = rec[r].q;
+                        const int dx = (int)(q & 0xffffu) - Xw;
+                        const int dy = (int)(q >> 16) - Yw;
+                        const G* b = lb - (dy * pitch + dx);
+                        P gr[2][2], gi[2][2];
 #pragma unroll
-                    for (int i = 0; i < RY; i++) {
-                        if (i < ilo || i > ihi) continue;
+                        for (int i = 0; i < 2; i++)
 #pragma unroll
-                        for (int j = 0; j < RX; j++) {
-                            if (j < jlo || j > jhi) continue;
-                            P gr, gi;
-                            Stage<G>::get(base[8 * i * pitch + 8 * j], gr, gi);
-                            if (sizeof(P) == sizeof(TA)) {
-                                acc[i][j].re = fma((TA)vr, (TA)gr, acc[i][j].re);
-                                acc[i][j].re = fma(-(TA)vi, (TA)gi, acc[i][j].re);
-                                acc[i][j].im = fma((TA)vr, (TA)gi, acc[i][j].im);
-                                acc[i][j].im = fma((TA)vi, (TA)gr, acc[i][j].im);
-                            } else {  // float products summed in double
-                                acc[i][j].re += (TA)(vr * gr - vi * gi);
-                                acc[i][j].im += (TA)(vr * gi + gr * vi);
+                            for (int j = 0; j < 2; j++)
+                                Stage<G>::get(b[8 * i * pitch + 8 * j], gr[i][j], gi[i][j]);
+                        mac4<P, TA>(acc, (P)rec[r].vr, (P)rec[r].vi, gr, gi);
+                    }
+                } else {
+                    for (int r = r0; r < r1; r++) {
+                        const uint32_t q = rec[r].q;
+                        const int c0 = S - 1 - ((int)(q & 0xffffu) - Xw);
+                        const int rr0 = S - 1 - ((int)(q >> 16) - Yw);
+                        P gr[2][2], gi[2][2];
+#pragma unroll
+                        for (int i = 0; i < 2; i++)
+#pragma unroll
+                            for (int j = 0; j < 2; j++) {
+                                const int c = c0 + lx + 8 * j, rr = rr0 + ly + 8 * i;
+                                gr[i][j] = gi[i][j] = (P)0;
+                                if ((unsigned)c < (unsigned)S && (unsigned)rr < (unsigned)S) {
+                                    const int64_t t = (int64_t)rr * S + c;
+                                    Stage<G>::get(Stage<G>::make((double)gk[2 * t], (double)gk[2 * t + 1]),
+                                                  gr[i][j], gi[i][j]);
+                                }
                             }
-                        }
+                        mac4<P, TA>(acc, (P)rec[r].vr, (P)rec[r].vi, gr, gi);
                     }
                 }
             }
         }
         // flush
 #pragma unroll
-        for (int i = 0; i < RY; i++) {
+        for (int i = 0; i < 2; i++) {
             const int y = Yw + 8 * i + ly;
             if (y >= g.N) continue;
 #pragma unroll
-            for (int j = 0; j < RX; j++) {
+            for (int j = 0; j < 2; j++) {
                 const int x = Xw + 8 * j + lx;
                 if (x >= g.N) continue;
                 const int64_t p = (int64_t)y * g.N + x;
@@ -444,11 +493,11 @@ __global__ __launch_bounds__(64 * WX * WY) void k_owner(const JobDesc* __restric
 __global__ void k_set_desc(JobDesc* dst, JobDesc d) { *dst = d; }
 
 // ---- host side ------------------------------------------------------------------
-template <int RX, int RY, int WX, int WY, typename G, typename TA, typename R, typename TI,
+template <int WX, int WY, typename G, typename TA, typename R, typename TI,
           typename TO, bool PLANAR, int GM>
 int run_owner(const OwnerLaunch& L, hipStream_t s)
 {
-    auto kern = k_owner<RX, RY, WX, WY, G, TA, R, TI, TO, PLANAR, GM>;
+    auto kern = k_owner<WX, WY, G, TA, R, TI, TO, PLANAR, GM>;
     MG_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)L.lds));
     int per_cu = 0;
@@ -463,18 +512,13 @@ int run_owner(const OwnerLaunch& L, hipStream_t s)
     return MIGRID_OK;
 }
 
-// Tile shapes: T = 64 uses 2x2 waves of 4x4 blocks (2x4 waves of 4x2 blocks
-// when the staged kernel is FP64, which allows one workgroup per CU);
-// T = 32 2x2 waves of 2x2 blocks; T = 16 2x2 waves of one block.
+// Tile shapes: a workgroup of (T/16)^2 waves, each owning a 16 x 16 block.
 template <typename G, typename TA, typename R, typename TI, typename TO, bool PLANAR, int GM>
 int run_shape(int T, const OwnerLaunch& L, hipStream_t s)
 {
-    if (T == 64) {
-        if (sizeof(G) == 16) return run_owner<4, 2, 2, 4, G, TA, R, TI, TO, PLANAR, GM>(L, s);
-        return run_owner<4, 4, 2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);
-    }
-    if (T == 32) return run_owner<2, 2, 2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);
-    if (T == 16) return run_owner<1, 1, 2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);
+    if (T == 64) return run_owner<4, 4, G, TA, R, TI, TO, PLANAR, GM>(L, s);
+    if (T == 32) return run_owner<2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);
+    if (T == 16) return run_owner<1, 1, G, TA, R, TI, TO, PLANAR, GM>(L, s);
     return fail(MIGRID_ENOTSUP, "tiled: tile edge %d (16, 32 or 64 supported)", T);
 }
 
@@ -538,12 +582,8 @@ int dispatch_owner(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
 TileCfg pick_tile(int S, int tile_opt)
 {
     TileCfg c;
-    c.T = tile_opt > 0 ? tile_opt : (S >= 48 ? 64 : 32);
-    // bin edge: the largest power of two <= max(S, 8) that divides T
-    int h = 8;
-    while (h * 2 <= S && h * 2 <= c.T) h *= 2;
-    c.H = h;
-    while (c.T % c.H) c.H /= 2;
+    c.T = tile_opt > 0 ? tile_opt : 64;
+    c.H = 16;  // bins match the 16 x 16 block a wave owns
     return c;
 }
 
@@ -623,7 +663,7 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.item_cap = item_cap;
     p.target = o.strategy == MIGRID_TILED ? (o.vis_per_wave > 0 ? o.vis_per_wave : 4096) : 0;
     const size_t gsz = p.key.stage == kStageBf16 ? 4 : p.key.stage == kStageF32 ? 8 : 16;
-    p.lds = (size_t)(p.smax + 2 * kPad) * (p.smax + 2 * kPad) * gsz;
+    p.lds = (size_t)stage_rows(p.smax) * stage_pitch(p.smax) * gsz;
     return MIGRID_OK;
 }
 
