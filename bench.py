@@ -141,6 +141,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=40000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variants", default="fp64_plain,fp64_tiled,fp32_tiled,bf16_tiled")
+    ap.add_argument("--tile", type=int, default=0, help="tiled: tile edge 16/32/64 (0 auto)")
+    ap.add_argument("--vpw", type=int, default=0, help="tiled: visibilities per work item")
+    ap.add_argument("--stage-min", type=int, default=0, help="tiled: staging threshold")
+    ap.add_argument("--wg-per-cu", type=int, default=0, help="tiled: workgroups per CU")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported "
@@ -169,10 +173,14 @@ def main():
         "fp32_tiled": dict(strategy="tiled", arith=migrid.ARITH_F32),
         "bf16_tiled": dict(strategy="tiled", arith=migrid.ARITH_F32_BF16GCF),
     }
+    tune = dict(tile=args.tile, vis_per_wave=args.vpw, stage_min=args.stage_min,
+                wg_per_cu=args.wg_per_cu)
     results = {}
     ref_grid = None
     for name in args.variants.split(","):
-        kw = variants[name]
+        kw = dict(variants[name])
+        if kw["strategy"] == "tiled":
+            kw.update(tune)
 
         def step(kw=kw):
             ctx.grid(grid, d_uv, d_vals, d_gcf, N, S, **kw)

@@ -84,7 +84,11 @@ typedef struct migrid_opts {
     int32_t vis_per_wave;   /* TILED work-item size, 0 = auto */
     double gcf_sigma;       /* ANALYTIC: Gaussian sigma in pixels (0: S/6) */
     double gcf_beta;        /* ANALYTIC: quadratic phase coefficient */
-    int32_t reserved[8];
+    int32_t stage_min;      /* TILED tuning: a sub-kernel with fewer visibilities
+                               for a tile is read from HBM instead of staged in
+                               LDS (0 = auto) */
+    int32_t wg_per_cu;      /* TILED tuning: persistent workgroups per CU (0 = auto) */
+    int32_t reserved[6];
 } migrid_opts;
 
 /* Fill *o with the defaults (scatter, default layout, table GCF, fresh grid). */

@@ -276,12 +276,34 @@ __host__ __device__ constexpr int stage_pitch(int S)
 {
     return stage_rows(S) + ((8 - stage_rows(S) % 32) + 32) % 32;
 }
-constexpr int kStageMin = 32;  // fewer visibilities for a k: read the table from HBM
+constexpr int kStageMinDefault = 4;  // fewer visibilities for a k: read the table from HBM
 
 template <typename R>
 using ConstPtr = const __attribute__((address_space(4))) R*;  // scalar-loadable
 template <typename R>
 using GlobalPtr = const __attribute__((address_space(1))) R*;
+
+// Element loads from the global address space (member-wise: the staged types
+// cannot be copied from an address-space-qualified object).
+template <typename G>
+__device__ __forceinline__ G gload(GlobalPtr<G> p, int64_t i);
+template <>
+__device__ __forceinline__ Bf16x2 gload<Bf16x2>(GlobalPtr<Bf16x2> p, int64_t i)
+{
+    Bf16x2 b;
+    b.w = p[i].w;
+    return b;
+}
+template <>
+__device__ __forceinline__ float2 gload<float2>(GlobalPtr<float2> p, int64_t i)
+{
+    return make_float2(p[i].x, p[i].y);
+}
+template <>
+__device__ __forceinline__ double2 gload<double2>(GlobalPtr<double2> p, int64_t i)
+{
+    return make_double2(p[i].x, p[i].y);
+}
 
 // acc[i][j] += v * g[i][j] for the 2 x 2 pixels of a lane: 4 FMAs per tap, or
 // float products summed in double for the <double2,float2> pair.
@@ -309,9 +331,14 @@ __device__ __forceinline__ void mac4(Cplx<TA> (&acc)[2][2], P vr, P vi, const P 
 // owns pixels (Xw + lx + 8j, Yw + ly + 8i), i, j in {0, 1}.
 template <int WX, int WY, typename G, typename TA, typename R, typename TI, typename TO,
           bool PLANAR, int GM>
-__global__ __launch_bounds__(64 * WX * WY) void k_owner(const JobDesc* __restrict__ jobs,
+// Register budget: two workgroups per CU need 8 waves per SIMD at T = 64
+// (the 4- and 8-byte staged kernels leave room for two in LDS).
+__global__ __launch_bounds__(64 * WX * WY)
+__attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
+    const JobDesc* __restrict__ jobs,
                                                         const Item* __restrict__ items,
-                                                        QueueState* __restrict__ qs, int smax)
+                                                        QueueState* __restrict__ qs, int smax,
+                                                        int stage_min)
 {
     using P = typename Stage<G>::P;  // product type
     constexpr int NT = 64 * WX * WY;
@@ -343,7 +370,7 @@ __global__ __launch_bounds__(64 * WX * WY) void k_owner(const JobDesc* __restric
         const TileGeom g = jobs[job].g;
         ConstPtr<R> rec = (ConstPtr<R>)(uintptr_t)jobs[job].rec;
         ConstPtr<int32_t> offs = (ConstPtr<int32_t>)(uintptr_t)jobs[job].offs;
-        const TI* __restrict__ gcf = (const TI*)jobs[job].gcf;
+        const G* __restrict__ gtab = (const G*)jobs[job].gcf;  // table in the staged type
         TO* __restrict__ grid = (TO*)jobs[job].grid;
         TO* __restrict__ grid_lo = (TO*)jobs[job].grid_lo;
         const double sigma = jobs[job].sigma, beta = jobs[job].beta;
@@ -384,21 +411,55 @@ __global__ __launch_bounds__(64 * WX * WY) void k_owner(const JobDesc* __restric
 #pragma unroll
             for (int j = 0; j < 2; j++) acc[i][j].re = acc[i][j].im = (TA)0;
 
-        for (int k = klo; k <= khi; k++) {
-            int nk = 0;
-            for (int by = Byl; by <= Byh; by++) {
-                const int64_t key = ((int64_t)k * g.nb + by) * g.nb;
-                nk += offs[key + Bxh + 1] - offs[key + Bxl];
+        // lane k counts the workgroup's visibilities for sub-kernel k; the
+        // ballots give the non-empty and the staged sub-kernels of this item
+        uint64_t nz_mask, st_mask;
+        {
+            int cnt = 0;
+            if (lane >= klo && lane <= khi)
+                for (int by = Byl; by <= Byh; by++) {
+                    const int64_t key = ((int64_t)lane * g.nb + by) * g.nb;
+                    cnt += offs[key + Bxh + 1] - offs[key + Bxl];
+                }
+            nz_mask = __ballot(cnt > 0);
+            st_mask = GM != 0 ? nz_mask : __ballot(cnt >= stage_min && cnt > 0);
+        }
+        // sub-kernel rows this wave stages: w, w + NW, ... (one column per lane);
+        // the next staged sub-kernel is fetched into registers while the
+        // current one is in use
+        constexpr int NW = WX * WY;
+        constexpr bool PF = GM == 0 && NW >= 8;
+        constexpr int PR = PF ? (64 + NW - 1) / NW : 1;
+        G pf[PR];
+        auto fetch_rows = [&](int kk) {
+            GlobalPtr<G> src = (GlobalPtr<G>)(uintptr_t)(gtab + (int64_t)kk * S * S);
+#pragma unroll
+            for (int p = 0; p < PR; p++) {
+                const int r = w + p * NW;
+                if (r < S && lane < S) pf[p] = gload<G>(src, r * S + lane);
             }
-            if (nk == 0) continue;
-            const bool staged = GM != 0 || nk >= kStageMin;
+        };
+        if (PF && st_mask) fetch_rows(__builtin_ctzll(st_mask));
+
+        for (uint64_t todo = nz_mask; todo; todo &= todo - 1) {
+            const int k = __builtin_ctzll(todo);
+            const bool staged = (st_mask >> k) & 1;
             if (staged && GM != 1) {
                 __syncthreads();  // previous sub-kernel no longer read
-                if (GM == 0) {
-                    const TI* gk = gcf + 2 * (int64_t)k * S * S;
-                    for (int e = tid; e < S * S; e += NT)
-                        sg[(e / S + B) * pitch + e % S + B] =
-                            Stage<G>::make((double)gk[2 * e], (double)gk[2 * e + 1]);
+                if (PF) {
+#pragma unroll
+                    for (int p = 0; p < PR; p++) {
+                        const int r = w + p * NW;
+                        if (r < S && lane < S) sg[(r + B) * pitch + lane + B] = pf[p];
+                    }
+                    const uint64_t later = st_mask & ~((2ull << k) - 1);
+                    if (later) fetch_rows(__builtin_ctzll(later));
+                } else if (GM == 0) {
+                    // wave w stages rows w, w + NW, ...; lanes walk the columns
+                    GlobalPtr<G> gk = (GlobalPtr<G>)(uintptr_t)(gtab + (int64_t)k * S * S);
+                    for (int r = w; r < S; r += NW)
+                        for (int c = lane; c < S; c += 64)
+                            sg[(r + B) * pitch + c + B] = gload<G>(gk, r * S + c);
                 } else {
                     // separable analytic GCF: G[r][c] = f(dy_r) * f(dx_c),
                     // f(d) = exp(-d^2/(2 sigma^2)) * exp(i beta d^2)
@@ -423,7 +484,7 @@ __global__ __launch_bounds__(64 * WX * WY) void k_owner(const JobDesc* __restric
                 }
                 __syncthreads();
             }
-            GlobalPtr<TI> gk = (GlobalPtr<TI>)(uintptr_t)(gcf + 2 * (int64_t)k * S * S);
+            GlobalPtr<G> gk = (GlobalPtr<G>)(uintptr_t)(gtab + (int64_t)k * S * S);
             for (int by = byl; by <= byh; by++) {
                 const int64_t key = ((int64_t)k * g.nb + by) * g.nb;
                 const int r0 = offs[key + bxl];
@@ -432,7 +493,7 @@ __global__ __launch_bounds__(64 * WX * WY) void k_owner(const JobDesc* __restric
                     // per lane: (row, column) of block (0,0) for q = (Xw, Yw),
                     // shifted per visibility by a wave-uniform offset
                     const G* lb = sg + lane_off + (S - 1 + B) * pitch + (S - 1 + B);
-#pragma unroll 4
+#pragma unroll 2
                     for (int r = r0; r < r1; r++) {
                         const uint32_t q = rec[r].q;
                         const int dx = (int)(q & 0xffffu) - Xw;
@@ -460,8 +521,7 @@ __global__ __launch_bounds__(64 * WX * WY) void k_owner(const JobDesc* __restric
                                 gr[i][j] = gi[i][j] = (P)0;
                                 if ((unsigned)c < (unsigned)S && (unsigned)rr < (unsigned)S) {
                                     const int64_t t = (int64_t)rr * S + c;
-                                    Stage<G>::get(Stage<G>::make((double)gk[2 * t], (double)gk[2 * t + 1]),
-                                                  gr[i][j], gi[i][j]);
+                                    Stage<G>::get(gload<G>(gk, t), gr[i][j], gi[i][j]);
                                 }
                             }
                         mac4<P, TA>(acc, (P)rec[r].vr, (P)rec[r].vi, gr, gi);
@@ -492,6 +552,41 @@ __global__ __launch_bounds__(64 * WX * WY) void k_owner(const JobDesc* __restric
 // Writes one job descriptor (passed by value, so the launch owns a copy).
 __global__ void k_set_desc(JobDesc* dst, JobDesc d) { *dst = d; }
 
+// The GCF table in the staged type (FP32 or BF16 pairs), converted once per
+// call so staging and the direct path move 8- or 4-byte elements.
+template <typename TI, typename G>
+__global__ __launch_bounds__(256) void k_convert_gcf(const TI* __restrict__ src, G* __restrict__ dst,
+                                                     int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = Stage<G>::make((double)src[2 * i], (double)src[2 * i + 1]);
+}
+
+int launch_convert(const TiledJob& t, int stage, char* arena, hipStream_t s)
+{
+    const int64_t n = (int64_t)kSubKernels * t.a.S * t.a.S;
+    const int blocks = (int)std::min<int64_t>(2048, (n + 255) / 256);
+    void* dst = arena + t.off_gtab;
+    const bool in64 = t.a.in_type == MIGRID_F64;
+    if (stage == kStageBf16) {
+        if (in64)
+            hipLaunchKernelGGL((k_convert_gcf<double, Bf16x2>), dim3(blocks), dim3(256), 0, s,
+                               (const double*)t.a.gcf, (Bf16x2*)dst, n);
+        else
+            hipLaunchKernelGGL((k_convert_gcf<float, Bf16x2>), dim3(blocks), dim3(256), 0, s,
+                               (const float*)t.a.gcf, (Bf16x2*)dst, n);
+    } else if (stage == kStageF32) {
+        hipLaunchKernelGGL((k_convert_gcf<double, float2>), dim3(blocks), dim3(256), 0, s,
+                           (const double*)t.a.gcf, (float2*)dst, n);
+    } else {
+        hipLaunchKernelGGL((k_convert_gcf<float, double2>), dim3(blocks), dim3(256), 0, s,
+                           (const float*)t.a.gcf, (double2*)dst, n);
+    }
+    MG_HIP(hipGetLastError());
+    return MIGRID_OK;
+}
+
 // ---- host side ------------------------------------------------------------------
 template <int WX, int WY, typename G, typename TA, typename R, typename TI,
           typename TO, bool PLANAR, int GM>
@@ -506,8 +601,9 @@ int run_owner(const OwnerLaunch& L, hipStream_t s)
     int dev = 0, ncu = 0;
     MG_HIP(hipGetDevice(&dev));
     MG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (L.wg_per_cu > 0) per_cu = L.wg_per_cu;
     hipLaunchKernelGGL(kern, dim3(ncu * per_cu), dim3(64 * WX * WY), L.lds, s, L.jobs, L.items,
-                       L.qs, L.smax);
+                       L.qs, L.smax, L.stage_min > 0 ? L.stage_min : kStageMinDefault);
     MG_HIP(hipGetLastError());
     return MIGRID_OK;
 }
@@ -649,6 +745,14 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         off += align256(((size_t)t.bg.nkeys + 1) * 4);
         t.off_split = off;
         off += align256((size_t)ntiles);
+        // a table in the staged type unless the caller's table already is one
+        t.convert = k.gm == 0 && !((k.stage == kStageF64 && x.in_type == MIGRID_F64) ||
+                                   (k.stage == kStageF32 && x.in_type == MIGRID_F32));
+        if (t.convert) {
+            const size_t gsz = k.stage == kStageBf16 ? 4 : k.stage == kStageF32 ? 8 : 16;
+            t.off_gtab = off;
+            off += align256((size_t)kSubKernels * x.S * x.S * gsz);
+        }
         item_cap += ntiles * kSubKernels;
         p.bin_bytes = std::max(p.bin_bytes, bin_scratch_bytes(x.npts, t.bg));
         p.smax = std::max(p.smax, x.S);
@@ -662,6 +766,8 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.arena_bytes = off;
     p.item_cap = item_cap;
     p.target = o.strategy == MIGRID_TILED ? (o.vis_per_wave > 0 ? o.vis_per_wave : 4096) : 0;
+    p.stage_min = o.stage_min;
+    p.wg_per_cu = o.wg_per_cu;
     const size_t gsz = p.key.stage == kStageBf16 ? 4 : p.key.stage == kStageF32 ? 8 : 16;
     p.lds = (size_t)stage_rows(p.smax) * stage_pitch(p.smax) * gsz;
     return MIGRID_OK;
@@ -676,7 +782,7 @@ void tiled_descs(const TiledPlan& p, char* arena, std::vector<JobDesc>& d)
         d[j].rec = arena + t.off_rec;
         d[j].offs = (const int32_t*)(arena + t.off_offs);
         d[j].split = (const uint8_t*)(arena + t.off_split);
-        d[j].gcf = t.a.gcf;
+        d[j].gcf = t.convert ? (const void*)(arena + t.off_gtab) : t.a.gcf;
         d[j].grid = t.a.grid;
         d[j].grid_lo = t.a.grid_lo;
         d[j].sigma = t.a.sigma;
@@ -704,6 +810,7 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
         const GridArgs& a = t.a;
         int32_t* perm = (int32_t*)(arena + t.off_perm);
         int32_t* offs = (int32_t*)(arena + t.off_offs);
+        if (t.convert) MG_TRY(launch_convert(t, p.key.stage, arena, s));
         MG_TRY(bin_sort(t.bg, a.in_type, a.uv, a.npts, bin_scratch, perm, offs, s));
         const int32_t* count = offs + t.bg.nkeys;
         if (a.npts > 0) {
@@ -733,6 +840,8 @@ int tiled_execute(const TiledPlan& p, char* arena, hipStream_t s)
     L.qs = (QueueState*)(arena + p.off_qs);
     L.smax = p.smax;
     L.lds = p.lds;
+    L.stage_min = p.stage_min;
+    L.wg_per_cu = p.wg_per_cu;
     return dispatch_owner(p.key, L, s);
 }
 

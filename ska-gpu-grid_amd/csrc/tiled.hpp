@@ -93,6 +93,8 @@ struct TiledJob {
     bool rec64 = false;
     int vt = 0;
     size_t off_rec = 0, off_perm = 0, off_offs = 0, off_split = 0;  // in the arena
+    bool convert = false;  // the GCF table is converted to the staged type
+    size_t off_gtab = 0;
 };
 
 // Device layout of one owner launch over n jobs: per-job records, bin-sort
@@ -106,6 +108,8 @@ struct TiledPlan {
     int64_t target = 0;  // visibilities per work item (0: one item per tile)
     int smax = 0;
     size_t lds = 0;
+    int stage_min = 0;  // tuning: staging threshold (0: default)
+    int wg_per_cu = 0;  // tuning: persistent workgroups per CU (0: occupancy)
 };
 
 struct OwnerLaunch {
@@ -113,6 +117,8 @@ struct OwnerLaunch {
     const Item* items;
     QueueState* qs;
     int smax;
+    int stage_min;  // staging threshold (0: default)
+    int wg_per_cu;  // persistent workgroups per CU (0: occupancy limit)
     size_t lds;
 };
 

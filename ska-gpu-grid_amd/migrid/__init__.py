@@ -49,7 +49,9 @@ class Opts(ctypes.Structure):
         ("vis_per_wave", ctypes.c_int32),
         ("gcf_sigma", ctypes.c_double),
         ("gcf_beta", ctypes.c_double),
-        ("reserved", ctypes.c_int32 * 8),
+        ("stage_min", ctypes.c_int32),
+        ("wg_per_cu", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -131,8 +133,10 @@ def version():
 
 def make_opts(strategy="scatter", layout=LAYOUT_DEFAULT, gcf_mode=GCF_TABLE, arith=ARITH_NATIVE,
               accumulate=False, debug_coverage=False, tile=0, vis_per_wave=0, sigma=0.0,
-              beta=0.05):
+              beta=0.05, stage_min=0, wg_per_cu=0):
     o = Opts()
+    o.stage_min = stage_min
+    o.wg_per_cu = wg_per_cu
     lib().migrid_opts_default(ctypes.byref(o))
     o.strategy = STRATEGIES[strategy] if isinstance(strategy, str) else int(strategy)
     o.layout = layout
