@@ -71,6 +71,20 @@ def gen_clustered(npts, N, S, mean_r=800.0, seed=1541618):
     return uv, vals
 
 
+def gen_uniform(npts, N, S, seed=1541618):
+    """Uniform uv over the grid interior (every window on the grid)."""
+    h = S // 2
+    c = np.arange(npts, dtype=np.uint64) * np.uint64(4)
+    lo, hi = h - 1.0, N - h - 1.0
+    u = (lo + (hi - lo) * splitmix_u01(seed, c)).astype(np.float32).astype(np.float64)
+    v = (lo + (hi - lo) * splitmix_u01(seed, c + np.uint64(1))).astype(np.float32).astype(np.float64)
+    u1 = 1.0 - splitmix_u01(seed, c + np.uint64(2))
+    u2 = splitmix_u01(seed, c + np.uint64(3))
+    rr = np.sqrt(-2.0 * np.log(u1))
+    vals = np.stack([rr * np.cos(2 * math.pi * u2), rr * np.sin(2 * math.pi * u2)], 1)
+    return np.stack([u, v], 1), vals
+
+
 def config_gcf(S, beta=0.05):
     sigma = S / 6.0
     sy, sx, r, c = np.meshgrid(np.arange(8), np.arange(8), np.arange(S), np.arange(S), indexing="ij")
@@ -145,6 +159,7 @@ def main():
     ap.add_argument("--vpw", type=int, default=0, help="tiled: visibilities per work item")
     ap.add_argument("--stage-min", type=int, default=0, help="tiled: staging threshold")
     ap.add_argument("--wg-per-cu", type=int, default=0, help="tiled: workgroups per CU")
+    ap.add_argument("--dist", default="clustered", choices=["clustered", "uniform"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported "
@@ -156,7 +171,7 @@ def main():
 
     N, S, npts = args.N, args.S, args.npts
     t0 = time.perf_counter()
-    uv, vals = gen_clustered(npts, N, S)
+    uv, vals = gen_clustered(npts, N, S) if args.dist == "clustered" else gen_uniform(npts, N, S)
     gcf = config_gcf(S)
     gen_s = time.perf_counter() - t0
     print(f"# generated {npts} visibilities in {gen_s:.1f} s", file=sys.stderr, flush=True)

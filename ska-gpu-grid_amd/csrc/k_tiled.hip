@@ -35,6 +35,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #include "bin.hpp"
 #include "device_common.hpp"
@@ -110,7 +112,7 @@ constexpr int kValDebug = 3;    // (1, n): the DEBUG1 coverage map
 template <typename TI, typename R, int VT>
 __global__ __launch_bounds__(256) void k_pack(const TI* __restrict__ uv, const TI* __restrict__ vals,
                                               const int32_t* __restrict__ perm,
-                                              const int32_t* __restrict__ count_p, int S,
+                                              const int32_t* __restrict__ count_p, int S, int pitch,
                                               const TI* __restrict__ gcf, R* __restrict__ rec)
 {
     const int64_t count = *count_p;
@@ -157,7 +159,7 @@ __global__ __launch_bounds__(256) void k_pack(const TI* __restrict__ uv, const T
         }
         R r;
         r.q = qx | (qy << 16);
-        r.n = (uint32_t)n;
+        r.off = qy * (uint32_t)pitch + qx;
         r.vr = (decltype(r.vr))vr;
         r.vi = (decltype(r.vi))vi;
         rec[j] = r;
@@ -305,6 +307,21 @@ __device__ __forceinline__ double2 gload<double2>(GlobalPtr<double2> p, int64_t 
     return make_double2(p[i].x, p[i].y);
 }
 
+// acc += v * g: 4 FMAs, or float products summed in double (<double2,float2>).
+template <typename P, typename TA>
+__device__ __forceinline__ void mac1(Cplx<TA>& acc, P vr, P vi, P gr, P gi)
+{
+    if (sizeof(P) == sizeof(TA)) {
+        acc.re = fma((TA)vr, (TA)gr, acc.re);
+        acc.re = fma(-(TA)vi, (TA)gi, acc.re);
+        acc.im = fma((TA)vr, (TA)gi, acc.im);
+        acc.im = fma((TA)vi, (TA)gr, acc.im);
+    } else {
+        acc.re += (TA)(vr * gr - vi * gi);
+        acc.im += (TA)(vr * gi + gr * vi);
+    }
+}
+
 // acc[i][j] += v * g[i][j] for the 2 x 2 pixels of a lane: 4 FMAs per tap, or
 // float products summed in double for the <double2,float2> pair.
 template <typename P, typename TA>
@@ -441,9 +458,31 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
         };
         if (PF && st_mask) fetch_rows(__builtin_ctzll(st_mask));
 
+        // CSR bounds of this wave's bin rows for one sub-kernel, gathered by
+        // one vector load (lane 2t: start of row byl + t, lane 2t+1: its end)
+        // and fetched one sub-kernel ahead, so no scalar-memory round trip
+        // sits between consecutive sub-kernels
+        constexpr int NBY = 5;  // bin rows per wave for S <= 64
+        GlobalPtr<int32_t> offs_v = (GlobalPtr<int32_t>)(uintptr_t)jobs[job].offs;
+        auto load_bounds = [&](int kk) -> int {
+            int v = 0;
+            const int t = lane >> 1;
+            if (lane < 2 * NBY && byl + t <= byh) {
+                const int64_t key = ((int64_t)kk * g.nb + byl + t) * g.nb;
+                v = offs_v[(lane & 1) ? key + bxh + 1 : key + bxl];
+            }
+            return v;
+        };
+        int bnd_next = nz_mask ? load_bounds(__builtin_ctzll(nz_mask)) : 0;
+
         for (uint64_t todo = nz_mask; todo; todo &= todo - 1) {
             const int k = __builtin_ctzll(todo);
             const bool staged = (st_mask >> k) & 1;
+            const int bnd = bnd_next;
+            {
+                const uint64_t rest = todo & (todo - 1);
+                if (rest) bnd_next = load_bounds(__builtin_ctzll(rest));
+            }
             if (staged && GM != 1) {
                 __syncthreads();  // previous sub-kernel no longer read
                 if (PF) {
@@ -485,20 +524,19 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
                 __syncthreads();
             }
             GlobalPtr<G> gk = (GlobalPtr<G>)(uintptr_t)(gtab + (int64_t)k * S * S);
-            for (int by = byl; by <= byh; by++) {
-                const int64_t key = ((int64_t)k * g.nb + by) * g.nb;
-                const int r0 = offs[key + bxl];
-                const int r1 = offs[key + bxh + 1];
+#pragma unroll
+            for (int t = 0; t < NBY; t++) {
+                const int r0 = __builtin_amdgcn_readlane(bnd, 2 * t);
+                const int r1 = __builtin_amdgcn_readlane(bnd, 2 * t + 1);
+                if (r0 >= r1) continue;
                 if (staged) {
-                    // per lane: (row, column) of block (0,0) for q = (Xw, Yw),
-                    // shifted per visibility by a wave-uniform offset
-                    const G* lb = sg + lane_off + (S - 1 + B) * pitch + (S - 1 + B);
+                    // per lane: the staged element of block (0,0) for q = 0;
+                    // a visibility shifts it by its wave-uniform record offset
+                    // q_y * pitch + q_x (taps outside the window read zeros)
+                    const G* lb = sg + lane_off + (S - 1 + B + Yw) * pitch + (S - 1 + B + Xw);
 #pragma unroll 2
                     for (int r = r0; r < r1; r++) {
-                        const uint32_t q = rec[r].q;
-                        const int dx = (int)(q & 0xffffu) - Xw;
-                        const int dy = (int)(q >> 16) - Yw;
-                        const G* b = lb - (dy * pitch + dx);
+                        const G* b = lb - (int)rec[r].off;
                         P gr[2][2], gi[2][2];
 #pragma unroll
                         for (int i = 0; i < 2; i++)
@@ -601,6 +639,15 @@ int run_owner(const OwnerLaunch& L, hipStream_t s)
     int dev = 0, ncu = 0;
     MG_HIP(hipGetDevice(&dev));
     MG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (getenv("MIGRID_VERBOSE")) {
+        hipFuncAttributes fa;
+        MG_HIP(hipFuncGetAttributes(&fa, (const void*)kern));
+        fprintf(stderr,
+                "[migrid] owner launch: %d CUs x %d WG/CU (occupancy %d), %d threads, "
+                "lds dyn %zu + static %zu, vgpr %d, sgpr %d, spill %zu\n",
+                ncu, L.wg_per_cu > 0 ? L.wg_per_cu : per_cu, per_cu, 64 * WX * WY, L.lds,
+                (size_t)fa.sharedSizeBytes, fa.numRegs, 0, (size_t)fa.localSizeBytes);
+    }
     if (L.wg_per_cu > 0) per_cu = L.wg_per_cu;
     hipLaunchKernelGGL(kern, dim3(ncu * per_cu), dim3(64 * WX * WY), L.lds, s, L.jobs, L.items,
                        L.qs, L.smax, L.stage_min > 0 ? L.stage_min : kStageMinDefault);
@@ -638,7 +685,7 @@ void launch_pack(const GridArgs& a, int vt, const int32_t* perm, const int32_t* 
     const int blocks = (int)std::min<int64_t>(4096, (a.npts + 255) / 256 + 1);
 #define MG_PACK(VT)                                                                          \
     hipLaunchKernelGGL((k_pack<TI, R, VT>), dim3(blocks), dim3(256), 0, s, (const TI*)a.uv,  \
-                       (const TI*)a.vals, perm, count, a.S, (const TI*)a.gcf, rec)
+                       (const TI*)a.vals, perm, count, a.S, stage_pitch(a.S), (const TI*)a.gcf, rec)
     switch (vt) {
     case kValCompatS: MG_PACK(kValCompatS); break;
     case kValCompatI: MG_PACK(kValCompatI); break;

@@ -15,12 +15,12 @@ namespace migrid {
 // q = floor(coordinate) + S/2 (so every binned visibility has 0 <= q < 2^16).
 struct alignas(16) RecF32 {
     uint32_t q;
-    uint32_t n;
+    uint32_t off;  // q_y * pitch + q_x of the staged-kernel layout
     float vr, vi;
 };
 struct alignas(16) RecF64 {
     uint32_t q;
-    uint32_t n;
+    uint32_t off;
     uint32_t pad0, pad1;
     double vr, vi;
 };
