@@ -187,6 +187,35 @@ int migrid_ctx_set_profiling(migrid_ctx* ctx, int enable);
 int migrid_ctx_profile_read(migrid_ctx* ctx, double* ms_out, int cap);
 int migrid_ctx_profile_reset(migrid_ctx* ctx);
 
+/* ---- batched launch (many jobs, one persistent kernel) --------------------
+ * No counterpart in the reference, which grids one job per gridGPU call.
+ * A batch plans all jobs (which must share input/output types, layout, GCF
+ * mode and tile edge; sizes, supports and grids may differ) into one device
+ * arena at creation.  migrid_batch_run enqueues, on `stream`: the grid clears
+ * (unless opts->accumulate), every job's bin sort, record packing and work
+ * items, then ONE persistent owner-kernel launch over a shared device work
+ * queue.  migrid_batch_graph_launch captures that same sequence into a
+ * hipGraph on its first call (stream must not be NULL) and replays it after;
+ * new data may be written into the same job buffers between replays.  The
+ * job pointers given at creation must stay valid for the batch's lifetime. */
+typedef struct migrid_batch migrid_batch;
+
+int migrid_batch_create(migrid_ctx* ctx, const migrid_job* jobs, int njobs,
+                        const migrid_opts* opts, migrid_batch** batch);
+int migrid_batch_run(migrid_batch* batch, void* stream);
+int migrid_batch_graph_launch(migrid_batch* batch, void* stream);
+int migrid_batch_info(migrid_batch* batch, int64_t* arena_bytes, int32_t* tile, int32_t* njobs);
+int migrid_batch_destroy(migrid_batch* batch);
+
+/* Autotuner: times tile edge {16, 32, 64} x visibilities per wavefront
+ * {64, 128, 256} (a work item holds 64 x that many visibilities) on the given
+ * jobs, returns the fastest options in *best (and its time in *best_ms), and
+ * caches the choice per shape class (set of supports and grid sizes,
+ * arithmetic, types, visibility-count decade) for the life of the process.
+ * Synchronous. */
+int migrid_autotune(migrid_ctx* ctx, const migrid_job* jobs, int njobs, const migrid_opts* base,
+                    migrid_opts* best, double* best_ms, void* stream);
+
 /* Number of bins per axis for migrid_ctx_bin_sort. */
 int64_t migrid_bins_per_axis(int64_t img_dim, int64_t gcf_dim, int32_t bin);
 

@@ -79,6 +79,8 @@ EXPORTS = [
     "migrid_ctx_create", "migrid_ctx_destroy", "migrid_ctx_grid",
     "migrid_ctx_bin_sort", "migrid_bins_per_axis",
     "migrid_ctx_set_profiling", "migrid_ctx_profile_read", "migrid_ctx_profile_reset",
+    "migrid_batch_create", "migrid_batch_run", "migrid_batch_graph_launch", "migrid_batch_info",
+    "migrid_batch_destroy", "migrid_autotune",
 ]
 
 _lib = None
@@ -118,6 +120,14 @@ def lib():
     L.migrid_ctx_profile_reset.argtypes = [vp]
     L.migrid_bins_per_axis.argtypes = [i64, i64, i32]
     L.migrid_bins_per_axis.restype = i64
+    L.migrid_batch_create.argtypes = [vp, ctypes.POINTER(Job), ctypes.c_int, ctypes.POINTER(Opts),
+                                      ctypes.POINTER(vp)]
+    L.migrid_batch_run.argtypes = [vp, vp]
+    L.migrid_batch_graph_launch.argtypes = [vp, vp]
+    L.migrid_batch_info.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    L.migrid_batch_destroy.argtypes = [vp]
+    L.migrid_autotune.argtypes = [vp, ctypes.POINTER(Job), ctypes.c_int, ctypes.POINTER(Opts),
+                                  ctypes.POINTER(Opts), ctypes.POINTER(ctypes.c_double), vp]
     _lib = L
     return L
 
@@ -135,9 +145,9 @@ def make_opts(strategy="scatter", layout=LAYOUT_DEFAULT, gcf_mode=GCF_TABLE, ari
               accumulate=False, debug_coverage=False, tile=0, vis_per_wave=0, sigma=0.0,
               beta=0.05, stage_min=0, wg_per_cu=0):
     o = Opts()
+    lib().migrid_opts_default(ctypes.byref(o))
     o.stage_min = stage_min
     o.wg_per_cu = wg_per_cu
-    lib().migrid_opts_default(ctypes.byref(o))
     o.strategy = STRATEGIES[strategy] if isinstance(strategy, str) else int(strategy)
     o.layout = layout
     o.gcf_mode = gcf_mode
@@ -300,3 +310,80 @@ class Context:
                                         offs.data_ptr(), ctypes.byref(cnt),
                                         ctypes.c_void_p(stream.cuda_stream)))
         return perm[: cnt.value], offs
+
+
+def make_job(grid, uv, vals, gcf, N, S, grid_lo=None):
+    """A Job structure over device tensors (see Context.grid for shapes)."""
+    import torch
+    j = Job()
+    j.in_type = F64 if uv.dtype == torch.float64 else F32
+    j.out_type = F64 if grid.dtype == torch.float64 else F32
+    for t in (uv, vals, grid) + ((gcf,) if gcf is not None else ()):
+        if not t.is_contiguous() or not t.is_cuda:
+            raise MigridError(-1, "device tensors must be contiguous and on the GPU")
+    j.uv = uv.data_ptr()
+    j.vals = vals.data_ptr()
+    j.gcf = gcf.data_ptr() if gcf is not None else None
+    j.grid = grid.data_ptr()
+    j.grid_lo = grid_lo.data_ptr() if grid_lo is not None else None
+    j.npts = uv.shape[0]
+    j.img_dim = N
+    j.gcf_dim = S
+    return j
+
+
+class Batch:
+    """Many jobs gridded by one persistent kernel launch (optionally replayed
+    from a captured hipGraph).  jobs: list of dicts with keys grid, uv, vals,
+    gcf, N, S (and optional grid_lo); the tensors must outlive the batch."""
+
+    def __init__(self, ctx, jobs, **opts):
+        self._keep = jobs
+        arr = (Job * len(jobs))(*[make_job(j["grid"], j["uv"], j["vals"], j.get("gcf"), j["N"], j["S"],
+                                           j.get("grid_lo")) for j in jobs])
+        self._jobs = arr
+        o = opts.pop("opts", None) or make_opts(**{"strategy": "tiled", **opts})
+        h = ctypes.c_void_p()
+        check(lib().migrid_batch_create(ctx.h, arr, len(jobs), ctypes.byref(o), ctypes.byref(h)))
+        self.h = h
+        self.ctx = ctx
+
+    def _stream(self, stream):
+        import torch
+        return ctypes.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
+
+    def run(self, stream=None):
+        check(lib().migrid_batch_run(self.h, self._stream(stream)))
+
+    def graph_launch(self, stream=None):
+        check(lib().migrid_batch_graph_launch(self.h, self._stream(stream)))
+
+    def info(self):
+        a, t, n = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        check(lib().migrid_batch_info(self.h, ctypes.byref(a), ctypes.byref(t), ctypes.byref(n)))
+        return {"arena_bytes": a.value, "tile": t.value, "njobs": n.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().migrid_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(ctx, jobs, stream=None, **opts):
+    """Returns (Opts, ms) of the fastest tile / work-item configuration."""
+    import torch
+    arr = (Job * len(jobs))(*[make_job(j["grid"], j["uv"], j["vals"], j.get("gcf"), j["N"], j["S"],
+                                       j.get("grid_lo")) for j in jobs])
+    base = make_opts(**{"strategy": "tiled", **opts})
+    best = Opts()
+    ms = ctypes.c_double()
+    s = ctypes.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
+    check(lib().migrid_autotune(ctx.h, arr, len(jobs), ctypes.byref(base), ctypes.byref(best),
+                                ctypes.byref(ms), s))
+    return best, ms.value
