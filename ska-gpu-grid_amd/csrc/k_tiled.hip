@@ -464,10 +464,14 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
         // sits between consecutive sub-kernels
         constexpr int NBY = 5;  // bin rows per wave for S <= 64
         GlobalPtr<int32_t> offs_v = (GlobalPtr<int32_t>)(uintptr_t)jobs[job].offs;
+        // A wave whose block lies past the grid edge (an edge tile) has no
+        // bins (its first bin index would be past the last one): its ranges
+        // stay empty, and it still takes part in the workgroup's barriers.
+        const bool wave_on = Xw < g.N && Yw < g.N && bxl <= bxh && byl <= byh;
         auto load_bounds = [&](int kk) -> int {
             int v = 0;
             const int t = lane >> 1;
-            if (lane < 2 * NBY && byl + t <= byh) {
+            if (wave_on && lane < 2 * NBY && byl + t <= byh) {
                 const int64_t key = ((int64_t)kk * g.nb + byl + t) * g.nb;
                 v = offs_v[(lane & 1) ? key + bxh + 1 : key + bxl];
             }
