@@ -136,6 +136,199 @@ def cpu_baseline(uv, vals, gcf, N, S, nsample):
                       f"single-threaded gridCPU restatement (reference loop order), {dt:.1f} s"}
 
 
+def gen_clustered_dev(torch, npts, N, S, mean_r, gen):
+    """Clustered uv and N(0,1) values drawn on the device (torch generator):
+    angle U[0, 2pi), radius Exp(mean_r) around the centre, redrawn until the
+    support window lies on the grid, uv rounded to float32 values."""
+    h = S // 2
+    uv = torch.empty((npts, 2), dtype=torch.float64, device="cuda")
+    done = 0
+    while done < npts:
+        m = int((npts - done) * 1.3) + 1024
+        th = 2 * math.pi * torch.rand(m, dtype=torch.float64, device="cuda", generator=gen)
+        r = -mean_r * torch.log1p(-torch.rand(m, dtype=torch.float64, device="cuda", generator=gen))
+        u = (N / 2 + r * torch.cos(th)).float().double()
+        v = (N / 2 + r * torch.sin(th)).float().double()
+        fu, fv = torch.floor(u), torch.floor(v)
+        ok = (fu - h + 1 >= 0) & (fu + h <= N - 1) & (fv - h + 1 >= 0) & (fv + h <= N - 1)
+        k = min(int(ok.sum()), npts - done)
+        uv[done:done + k] = torch.stack([u[ok][:k], v[ok][:k]], 1)
+        done += k
+    vals = torch.randn((npts, 2), dtype=torch.float64, device="cuda", generator=gen)
+    return uv, vals
+
+
+def batch_jobs(rng, njobs):
+    """Job shapes of the batched workload: visibilities log-uniform in
+    [50k, 2M], grid edge from {1024, 2048, 4096}, support from {8, 16, 32, 64}."""
+    n = np.exp(rng.uniform(math.log(5e4), math.log(2e6), njobs)).astype(np.int64)
+    Ns = rng.choice([1024, 2048, 4096], njobs)
+    Ss = rng.choice([8, 16, 32, 64], njobs)
+    return [(int(a), int(b), int(c)) for a, b, c in zip(Ns, Ss, n)]
+
+
+def main_batch(args, torch, migrid):
+    rng = np.random.default_rng(args.seed)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(args.seed)
+    shapes = batch_jobs(rng, args.jobs)
+    gcfs = {S: torch.from_numpy(config_gcf(S)).cuda() for S in (8, 16, 32, 64)}
+    jobs = []
+    for N, S, n in shapes:
+        uv, vals = gen_clustered_dev(torch, n, N, S, 800.0 * N / 8192, gen)
+        jobs.append(dict(grid=torch.empty((N, N, 2), dtype=torch.float64, device="cuda"),
+                         uv=uv, vals=vals, gcf=gcfs[S], N=N, S=S))
+    torch.cuda.synchronize()
+    total = sum(n for _, _, n in shapes)
+    print(f"# batch: {len(jobs)} jobs, {total} visibilities", file=sys.stderr, flush=True)
+    ctx = migrid.Context()
+    t0 = time.perf_counter()
+    best, best_ms = migrid.autotune(ctx, jobs, arith=migrid.ARITH_F32)
+    tune_s = time.perf_counter() - t0
+    print(f"# autotune: tile {best.tile} vis/item {best.vis_per_wave} {best_ms:.2f} ms "
+          f"({tune_s:.1f} s)", file=sys.stderr, flush=True)
+    b = migrid.Batch(ctx, jobs, opts=best)
+    stream = torch.cuda.Stream()  # graph capture needs a non-default stream
+    for _ in range(args.warmup):
+        b.graph_launch(stream)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record(stream)
+        b.graph_launch(stream)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    per = [e0.elapsed_time(e1) for e0, e1 in evs]
+    tiled = [j["grid"].clone() for j in jobs[:8]]
+    # the plain FP64 path: one scatter launch per job
+    t0 = time.perf_counter()
+    for j in jobs:
+        ctx.grid(j["grid"], j["uv"], j["vals"], j["gcf"], j["N"], j["S"], strategy="scatter")
+    torch.cuda.synchronize()
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    err = 0.0
+    for g, j in zip(tiled, jobs[:8]):
+        d = torch.linalg.vector_norm(g - j["grid"], dim=-1).max()
+        err = max(err, float(d / torch.linalg.vector_norm(j["grid"], dim=-1).max()))
+    out = {
+        "metric": "Mvis/s, 64 uneven gridding jobs in one hipGraph-captured persistent launch",
+        "value": total / (ms * 1e-3) / 1e6, "unit": "Mvis/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{len(jobs)} jobs: vis log-uniform [50k, 2M], grids "
+                               "{1024,2048,4096}^2, supports {8,16,32,64}, clustered uv, "
+                               "FP32 tiles into FP64 grids", "seed": args.seed, "npts_total": total},
+        "p50_ms": pct(per, 50), "p99_ms": pct(per, 99),
+        "autotune": {"tile": best.tile, "vis_per_item": best.vis_per_wave, "ms": best_ms,
+                     "seconds": tune_s},
+        "fp64_plain_one_launch_per_job": {"ms": plain_ms,
+                                          "mvis_per_s": total / (plain_ms * 1e-3) / 1e6},
+        "rel_max_vs_fp64_plain": err,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def main_serve(args, torch, migrid):
+    import threading
+    rng = np.random.default_rng(args.seed)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(args.seed)
+    gcfs = {S: torch.from_numpy(config_gcf(S)).cuda() for S in (8, 16, 32, 64)}
+    small = batch_jobs(rng, args.jobs)  # (N, S, max vis) per small job id
+    pools = {}
+    for jid, (N, S, _) in enumerate(small):
+        pools[jid] = gen_clustered_dev(torch, 2_000_000, N, S, 800.0 * N / 8192, gen) + (N, S)
+    nlarge = args.large_jobs
+    big = gen_clustered_dev(torch, 10_000_000, 8192, 64, 800.0, gen)
+    for k in range(nlarge):
+        pools[1000 + k] = big + (8192, 64)
+    torch.cuda.synchronize()
+    # Poisson arrivals; 10% of requests are 8192^2 / 10M visibilities
+    n_req = int(args.rate * args.duration * 1.2) + 16
+    arr = np.cumsum(rng.exponential(1.0 / args.rate, n_req))
+    arr = arr[arr < args.duration]
+    trace = []
+    for t in arr:
+        if rng.uniform() < 0.1:
+            jid, n, off = 1000 + int(rng.integers(nlarge)), 10_000_000, 0
+        else:
+            jid = int(rng.integers(len(small)))
+            n = int(math.exp(rng.uniform(math.log(5e4), math.log(2e6))))
+            off = int(rng.integers(0, 2_000_000 - n + 1))
+        trace.append((float(t), jid, n, off))
+    total = sum(n for _, _, n, _ in trace)
+    print(f"# serve: {len(trace)} requests over {args.duration} s, {total} visibilities",
+          file=sys.stderr, flush=True)
+
+    def req_args(jid, n, off):
+        uv, vals, N, S = pools[jid]
+        return uv[off:off + n], vals[off:off + n], gcfs[S], N, S
+
+    srv = migrid.Server(max_batch=args.max_batch, max_inflight=args.max_inflight)
+    # warm-up request on a job id outside the trace
+    srv.wait(srv.submit(-1, *req_args(0, 50_000, 0)))
+    srv.drop_job(-1)
+    st0 = srv.stats()
+    tickets = []
+    t_start = time.perf_counter()
+    for t, jid, n, off in trace:
+        now = time.perf_counter() - t_start
+        if t > now:
+            time.sleep(t - now)
+        tickets.append(srv.submit(jid, *req_args(jid, n, off)))
+        if len(tickets) % 2000 == 0:
+            print(f"# serve: {len(tickets)} submitted", file=sys.stderr, flush=True)
+    lat = [srv.wait(tk) for tk in tickets]
+    wall = time.perf_counter() - t_start
+    st = srv.stats()
+    srv.close()
+    batches = st["batches"] - st0["batches"]
+    gpu_ms = st["gpu_ms"] - st0["gpu_ms"]
+    # the plain FP64 path on a prefix of the same trace: FIFO, one scatter
+    # launch per request, accumulating into per-job grids
+    prefix = [r for r in trace if r[0] < args.plain_duration]
+    ctx = migrid.Context()
+    grids = {}
+    plain_lat = []
+    t_start = time.perf_counter()
+    for t, jid, n, off in prefix:
+        now = time.perf_counter() - t_start
+        if t > now:
+            time.sleep(t - now)
+        uv, vals, g, N, S = req_args(jid, n, off)
+        if jid not in grids:
+            grids[jid] = torch.zeros((N, N, 2), dtype=torch.float64, device="cuda")
+        ctx.grid(grids[jid], uv, vals, g, N, S, strategy="scatter", accumulate=True)
+        torch.cuda.synchronize()
+        plain_lat.append((time.perf_counter() - t_start - t) * 1e3)
+    plain_wall = time.perf_counter() - t_start
+    plain_vis = sum(n for _, _, n, _ in prefix)
+    out = {
+        "metric": "Mvis/s and p99 latency of the serving loop, Poisson 200 req/s",
+        "value": total / wall / 1e6, "unit": "Mvis/s", "n_gpus": 1, "steps": len(trace),
+        "warmup": 1, "ms_per_step": wall * 1e3 / len(trace), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"Poisson {args.rate} req/s for {args.duration} s; 90% of requests "
+                               f"on {len(small)} job ids with grids {{1024,2048,4096}}^2, supports "
+                               "{8,16,32,64}, vis log-uniform [50k, 2M]; 10% on "
+                               f"{nlarge} job ids of 8192^2, S=64, 10M vis; clustered uv; FP32 "
+                               "tiles into resident FP64 grids", "seed": args.seed,
+                   "requests": len(trace), "npts_total": total},
+        "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "max_ms": max(lat),
+        "batches": batches, "mean_requests_per_batch": len(trace) / max(batches, 1),
+        "gpu_busy_frac": gpu_ms / (wall * 1e3),
+        "offered_mvis_per_s": total / args.duration / 1e6,
+        "fp64_plain_one_launch_per_request": {
+            "requests": len(prefix), "trace_seconds": args.plain_duration,
+            "mvis_per_s": plain_vis / plain_wall / 1e6,
+            "p50_ms": pct(plain_lat, 50), "p99_ms": pct(plain_lat, 99)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(p):
@@ -160,6 +353,18 @@ def main():
     ap.add_argument("--stage-min", type=int, default=0, help="tiled: staging threshold")
     ap.add_argument("--wg-per-cu", type=int, default=0, help="tiled: workgroups per CU")
     ap.add_argument("--dist", default="clustered", choices=["clustered", "uniform"])
+    ap.add_argument("--mode", default="grid", choices=["grid", "batch", "serve"],
+                    help="grid: one 8192^2 job (default); batch: many jobs in one persistent "
+                         "launch; serve: Poisson request stream through the serving loop")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--jobs", type=int, default=64, help="batch/serve: job ids")
+    ap.add_argument("--large-jobs", type=int, default=4, help="serve: 8192^2 job ids")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: requests per second")
+    ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
+    ap.add_argument("--plain-duration", type=float, default=1.0,
+                    help="serve: seconds of the trace replayed on the plain FP64 path")
+    ap.add_argument("--max-batch", type=int, default=0)
+    ap.add_argument("--max-inflight", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported "
@@ -169,6 +374,10 @@ def main():
     import torch
     import migrid
 
+    if args.mode == "batch":
+        return main_batch(args, torch, migrid)
+    if args.mode == "serve":
+        return main_serve(args, torch, migrid)
     N, S, npts = args.N, args.S, args.npts
     t0 = time.perf_counter()
     uv, vals = gen_clustered(npts, N, S) if args.dist == "clustered" else gen_uniform(npts, N, S)

@@ -219,6 +219,62 @@ int migrid_autotune(migrid_ctx* ctx, const migrid_job* jobs, int njobs, const mi
 /* Number of bins per axis for migrid_ctx_bin_sort. */
 int64_t migrid_bins_per_axis(int64_t img_dim, int64_t gcf_dim, int32_t bin);
 
+/* ---- serving loop (continuous batching) -----------------------------------
+ * No counterpart in the reference, whose only entry is the blocking
+ * per-job gridGPU call (grid_gpu.cu:675-819).  A server owns a worker thread
+ * and two streams on one device.  Requests carry a job id; each job id owns a
+ * resident N x N grid (interleaved complex, out_type) that starts at zero and
+ * accumulates every request submitted for it until migrid_server_drop_job.
+ * Requests are admitted in arrival order while their device footprint fits
+ * hbm_budget; up to max_inflight batches of up to max_batch requests (one per
+ * job id) are queued on the device, each batch one persistent launch of the
+ * tiled gridder with cfg.opts (strategy and accumulate are forced).
+ * Thread-safe: any thread may submit, poll, wait, read and drop. */
+typedef struct migrid_server migrid_server;
+
+typedef struct migrid_server_cfg {
+    int32_t device;
+    int32_t in_type;      /* MIGRID_F64 or MIGRID_F32 uv/vals/gcf */
+    int32_t out_type;     /* resident grid element type */
+    int32_t max_batch;    /* requests per launch (0: 64) */
+    int32_t max_inflight; /* batches queued on the device, 1..8 (0: 2) */
+    int32_t reserved0;
+    int64_t hbm_budget;   /* bytes the server may hold (0: 90% of free HBM) */
+    migrid_opts opts;     /* arithmetic, tile, tuning; GCF mode */
+} migrid_server_cfg;
+
+typedef struct migrid_server_stats {
+    int64_t submitted, completed, failed, batches;
+    int64_t vis_completed;
+    int64_t hbm_in_use, hbm_budget, jobs_resident;
+    int64_t queued;  /* submitted or admitted, not yet on the device */
+    int64_t running; /* in batches on the device */
+    double gpu_ms;   /* summed device time of completed batches */
+} migrid_server_stats;
+
+#define MIGRID_SUBMIT_HOST 0   /* uv/vals/gcf in host memory, copied at submit */
+#define MIGRID_SUBMIT_DEVICE 1 /* device pointers, valid until the request completes */
+
+void migrid_server_cfg_default(migrid_server_cfg* cfg);
+int migrid_server_create(const migrid_server_cfg* cfg, migrid_server** server);
+/* Queues one request; *ticket identifies it.  gcf holds 64 x gcf_dim^2
+ * complex values as in gridGPU; a job id keeps its img_dim for life. */
+int migrid_server_submit(migrid_server* server, int64_t job_id, int64_t img_dim, int64_t gcf_dim,
+                         const void* gcf, const void* uv, const void* vals, int64_t npts,
+                         int32_t flags, int64_t* ticket);
+/* *done = 1 when the request has completed; returns its status then.
+ * latency_ms: submit to completion as seen by the worker. */
+int migrid_server_poll(migrid_server* server, int64_t ticket, int32_t* done, double* latency_ms);
+int migrid_server_wait(migrid_server* server, int64_t ticket, double* latency_ms);
+/* Waits for every request of the job submitted so far, then copies its grid
+ * (img_dim^2 complex of out_type) to host memory. */
+int migrid_server_read_grid(migrid_server* server, int64_t job_id, void* host_out);
+/* Waits like read_grid, then frees the job's grid. */
+int migrid_server_drop_job(migrid_server* server, int64_t job_id);
+int migrid_server_get_stats(migrid_server* server, migrid_server_stats* stats);
+/* Completes every submitted request, then frees everything. */
+int migrid_server_destroy(migrid_server* server);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -70,6 +70,25 @@ class Job(ctypes.Structure):
     ]
 
 
+class ServerCfg(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("in_type", ctypes.c_int32),
+        ("out_type", ctypes.c_int32),
+        ("max_batch", ctypes.c_int32),
+        ("max_inflight", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
+        ("hbm_budget", ctypes.c_int64),
+        ("opts", Opts),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "submitted", "completed", "failed", "batches", "vis_completed", "hbm_in_use", "hbm_budget",
+        "jobs_resident", "queued", "running")] + [("gpu_ms", ctypes.c_double)]
+
+
 # Every symbol include/migrid.h declares (checked by the CPU test suite).
 EXPORTS = [
     "migrid_opts_default", "migrid_last_error", "migrid_version",
@@ -81,6 +100,9 @@ EXPORTS = [
     "migrid_ctx_set_profiling", "migrid_ctx_profile_read", "migrid_ctx_profile_reset",
     "migrid_batch_create", "migrid_batch_run", "migrid_batch_graph_launch", "migrid_batch_info",
     "migrid_batch_destroy", "migrid_autotune",
+    "migrid_server_cfg_default", "migrid_server_create", "migrid_server_submit", "migrid_server_poll",
+    "migrid_server_wait", "migrid_server_read_grid", "migrid_server_drop_job",
+    "migrid_server_get_stats", "migrid_server_destroy",
 ]
 
 _lib = None
@@ -128,6 +150,16 @@ def lib():
     L.migrid_batch_destroy.argtypes = [vp]
     L.migrid_autotune.argtypes = [vp, ctypes.POINTER(Job), ctypes.c_int, ctypes.POINTER(Opts),
                                   ctypes.POINTER(Opts), ctypes.POINTER(ctypes.c_double), vp]
+    L.migrid_server_cfg_default.argtypes = [ctypes.POINTER(ServerCfg)]
+    L.migrid_server_cfg_default.restype = None
+    L.migrid_server_create.argtypes = [ctypes.POINTER(ServerCfg), ctypes.POINTER(vp)]
+    L.migrid_server_submit.argtypes = [vp, i64, i64, i64, vp, vp, vp, i64, i32, ctypes.POINTER(i64)]
+    L.migrid_server_poll.argtypes = [vp, i64, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double)]
+    L.migrid_server_wait.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double)]
+    L.migrid_server_read_grid.argtypes = [vp, i64, vp]
+    L.migrid_server_drop_job.argtypes = [vp, i64]
+    L.migrid_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+    L.migrid_server_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -387,3 +419,104 @@ def autotune(ctx, jobs, stream=None, **opts):
     check(lib().migrid_autotune(ctx.h, arr, len(jobs), ctypes.byref(base), ctypes.byref(best),
                                 ctypes.byref(ms), s))
     return best, ms.value
+
+
+# ---- serving loop -------------------------------------------------------------
+
+SUBMIT_HOST, SUBMIT_DEVICE = 0, 1
+
+
+class Server:
+    """Continuous-batching request loop over resident per-job grids.
+
+    submit() takes numpy arrays (copied at submit) or device torch tensors
+    (which must stay alive until the request completes); read_grid() returns
+    the job's accumulated N x N complex grid as numpy."""
+
+    def __init__(self, in_type=F64, out_type=F64, max_batch=0, max_inflight=0, hbm_budget=0,
+                 device=0, **opts):
+        c = ServerCfg()
+        lib().migrid_server_cfg_default(ctypes.byref(c))
+        c.device = device
+        c.in_type, c.out_type = in_type, out_type
+        c.max_batch, c.max_inflight, c.hbm_budget = max_batch, max_inflight, int(hbm_budget)
+        c.opts = make_opts(**{"strategy": "tiled", "arith": ARITH_F32, **opts})
+        self.cfg = c
+        h = ctypes.c_void_p()
+        check(lib().migrid_server_create(ctypes.byref(c), ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+        self._N = {}
+
+    def submit(self, job_id, uv, vals, gcf, N, S):
+        ctype = np.complex128 if self.cfg.in_type == F64 else np.complex64
+        rtype = np.float64 if self.cfg.in_type == F64 else np.float32
+        t = ctypes.c_int64()
+        if isinstance(uv, np.ndarray):
+            uv = np.ascontiguousarray(uv, dtype=rtype).reshape(-1, 2)
+            vals = np.ascontiguousarray(vals)
+            vals = (vals.view(rtype).reshape(-1, 2) if np.iscomplexobj(vals)
+                    else np.ascontiguousarray(vals, dtype=rtype).reshape(-1, 2))
+            g = np.ascontiguousarray(gcf)
+            g = g.astype(ctype) if np.iscomplexobj(g) else np.ascontiguousarray(g, dtype=rtype)
+            if g.size * g.itemsize != 64 * S * S * 2 * np.dtype(rtype).itemsize:
+                raise MigridError(-1, f"gcf must hold 64*S*S complex values (S={S})")
+            check(lib().migrid_server_submit(self.h, job_id, N, S, g.ctypes.data, uv.ctypes.data,
+                                             vals.ctypes.data, uv.shape[0], SUBMIT_HOST,
+                                             ctypes.byref(t)))
+        else:
+            for x in (uv, vals, gcf):
+                if not x.is_cuda or not x.is_contiguous():
+                    raise MigridError(-1, "device tensors must be contiguous and on the GPU")
+            check(lib().migrid_server_submit(self.h, job_id, N, S, gcf.data_ptr(), uv.data_ptr(),
+                                             vals.data_ptr(), uv.shape[0], SUBMIT_DEVICE,
+                                             ctypes.byref(t)))
+            self._keep[t.value] = (uv, vals, gcf)
+        self._N[job_id] = N
+        return t.value
+
+    def poll(self, ticket):
+        d, ms = ctypes.c_int32(), ctypes.c_double()
+        check(lib().migrid_server_poll(self.h, ticket, ctypes.byref(d), ctypes.byref(ms)))
+        if d.value:
+            self._keep.pop(ticket, None)
+        return bool(d.value), ms.value
+
+    def wait(self, ticket):
+        ms = ctypes.c_double()
+        rc = lib().migrid_server_wait(self.h, ticket, ctypes.byref(ms))
+        self._keep.pop(ticket, None)
+        check(rc)
+        return ms.value
+
+    def read_grid(self, job_id):
+        N = self._N[job_id]
+        out = np.empty((N, N), np.complex128 if self.cfg.out_type == F64 else np.complex64)
+        check(lib().migrid_server_read_grid(self.h, job_id, out.ctypes.data))
+        return out
+
+    def drop_job(self, job_id):
+        check(lib().migrid_server_drop_job(self.h, job_id))
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().migrid_server_get_stats(self.h, ctypes.byref(st)))
+        return {n: getattr(st, n) for n, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(lib().migrid_server_destroy(self.h))
+            self.h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -75,3 +75,33 @@ def test_make_opts_keeps_every_field():
     assert (o.strategy, o.tile, o.vis_per_wave, o.stage_min, o.wg_per_cu, o.accumulate) == \
         (migrid.TILED, 32, 777, 5, 2, 1)
     assert o.gcf_sigma == 3.0 and o.gcf_beta == 0.1
+
+
+def test_server_structs_and_defaults():
+    assert ctypes.sizeof(migrid.ServerCfg) == 6 * 4 + 8 + ctypes.sizeof(migrid.Opts)
+    assert ctypes.sizeof(migrid.ServerStats) == 11 * 8
+    c = migrid.ServerCfg()
+    migrid.lib().migrid_server_cfg_default(ctypes.byref(c))
+    assert (c.in_type, c.out_type, c.max_batch, c.max_inflight) == (migrid.F64, migrid.F64, 64, 2)
+    assert c.opts.strategy == migrid.TILED and c.opts.arith == migrid.ARITH_F32
+
+
+@pytest.mark.parametrize("field,value", [("in_type", 7), ("out_type", -1), ("max_inflight", 9),
+                                         ("max_batch", -2), ("hbm_budget", -5)])
+def test_server_config_rejected_before_any_device_call(field, value):
+    L = migrid.lib()
+    c = migrid.ServerCfg()
+    L.migrid_server_cfg_default(ctypes.byref(c))
+    setattr(c, field, value)
+    h = ctypes.c_void_p()
+    assert L.migrid_server_create(ctypes.byref(c), ctypes.byref(h)) == -1
+    assert not h.value and L.migrid_last_error().decode()
+
+
+def test_server_null_handles_rejected():
+    L = migrid.lib()
+    t = ctypes.c_int64()
+    assert L.migrid_server_submit(None, 1, 64, 8, None, None, None, 0, 0, ctypes.byref(t)) == -1
+    assert L.migrid_server_wait(None, 1, None) == -1
+    assert L.migrid_server_read_grid(None, 1, None) == -1
+    assert L.migrid_server_destroy(None) == 0

@@ -17,7 +17,7 @@ def build_dropin(force=True):
         return exe
     subprocess.run(["g++", "-std=c++14", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"),
                     os.path.join(ROOT, "tests", "cxx", "dropin_main.cpp"), "-L", LIBDIR, "-lmigrid",
-                    "-Wl,-rpath," + LIBDIR, "-o", exe], check=True)
+                    "-Wl,-rpath,$ORIGIN/../../ska-gpu-grid_amd/migrid", "-o", exe], check=True)
     return exe
 
 

@@ -1,0 +1,152 @@
+"""GPU tests of the serving loop (continuous batching over resident grids).
+
+Each result must be the same when its request is served alone and when it is
+served in a mix of concurrent requests: FP32 tile arithmetic into FP64 grids
+within 1e-5 relative max-norm of the FP64 oracle (and of each other), native
+FP64 arithmetic within 1e-12.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+import migrid
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(128, 8, 3000), (256, 16, 6000), (192, 32, 4000), (300, 64, 5000), (96, 10, 800),
+          (512, 64, 20000)]
+
+
+def torch():
+    import torch as t
+    return t
+
+
+def inputs(i, N, S, n, seed=0):
+    if i % 2:
+        uv, vals = O.gen_clustered(n, N, S, mean_r=N / 8, seed=seed + i)
+    else:
+        uv, vals = O.gen_uniform(n, -S / 2, N + S / 2, seed=seed + i)
+    return uv, vals, O.config_gcf(S)
+
+
+def ref(uv, vals, gcf, N, S):
+    p = O.grid_zz(uv, vals, N, gcf, S)
+    return p[0] + 1j * p[1]
+
+
+def test_result_is_the_same_alone_and_in_the_mix():
+    N, S, n = 256, 32, 8000
+    uv, vals, gcf = inputs(1, N, S, n, seed=40)
+    want = ref(uv, vals, gcf, N, S)
+    with migrid.Server() as srv:
+        srv.wait(srv.submit(7, uv, vals, gcf, N, S))
+        alone = srv.read_grid(7)
+    assert O.rel_errors(alone, want)[0] < 1e-5
+
+    # the same request among concurrent ones from several threads (device
+    # submissions, so a burst outpaces the device and shares launches)
+    t = torch()
+    others = [(100 + i, *SHAPES[i % len(SHAPES)]) for i in range(24)]
+    data = {jid: inputs(k, N2, S2, n2, seed=k) + (N2, S2) for k, (jid, N2, S2, n2) in enumerate(others)}
+    dev = {jid: [t.from_numpy(np.ascontiguousarray(x)).cuda() for x in d[:3]] for jid, d in data.items()}
+    t.cuda.synchronize()
+    with migrid.Server(max_batch=8) as srv:
+        tickets, lock = [], threading.Lock()
+
+        def client(ids):
+            for jid in ids:
+                N2, S2 = data[jid][3:]
+                tk = srv.submit(jid, *dev[jid], N2, S2)
+                with lock:
+                    tickets.append(tk)
+
+        ids = list(data)
+        th = [threading.Thread(target=client, args=(ids[k::3],)) for k in range(3)]
+        for c in th:
+            c.start()
+        mine = srv.submit(7, uv, vals, gcf, N, S)
+        for c in th:
+            c.join()
+        for tk in tickets + [mine]:
+            srv.wait(tk)
+        mixed = srv.read_grid(7)
+        st = srv.stats()
+        assert st["completed"] == len(others) + 1 and st["failed"] == 0
+        assert st["batches"] < st["completed"]  # requests shared launches
+        for jid, (u, v, g, N2, S2) in list(data.items())[:6]:
+            assert O.rel_errors(srv.read_grid(jid), ref(u, v, g, N2, S2))[0] < 1e-5, jid
+    assert O.rel_errors(mixed, want)[0] < 1e-5
+    assert O.rel_errors(mixed, alone)[0] < 1e-5
+
+
+def test_grids_accumulate_across_requests_of_a_job():
+    N, S = 192, 16
+    parts = [inputs(i, N, S, 3000 + 500 * i, seed=9) for i in range(3)]
+    want = sum(ref(u, v, g, N, S) for u, v, g in parts)
+    with migrid.Server(arith=migrid.ARITH_NATIVE) as srv:
+        ts = [srv.submit(3, u, v, g, N, S) for u, v, g in parts]
+        for t in ts:
+            srv.wait(t)
+        got = srv.read_grid(3)
+        assert O.rel_errors(got, want)[0] < 1e-12
+        srv.drop_job(3)
+        assert srv.stats()["jobs_resident"] == 0
+        # a dropped job starts again from zero
+        u, v, g = parts[0]
+        srv.wait(srv.submit(3, u, v, g, N, S))
+        assert O.rel_errors(srv.read_grid(3), ref(u, v, g, N, S))[0] < 1e-12
+
+
+def test_device_and_host_submissions_agree():
+    t = torch()
+    N, S, n = 300, 64, 5000
+    uv, vals, gcf = inputs(3, N, S, n, seed=5)
+    want = ref(uv, vals, gcf, N, S)
+    with migrid.Server(arith=migrid.ARITH_NATIVE) as srv:
+        srv.wait(srv.submit(1, uv, vals, gcf, N, S))
+        d = [t.from_numpy(np.ascontiguousarray(x)).cuda() for x in (uv, vals, gcf)]
+        t.cuda.synchronize()
+        srv.wait(srv.submit(2, *d, N, S))
+        a, b = srv.read_grid(1), srv.read_grid(2)
+    assert O.rel_errors(a, want)[0] < 1e-12
+    assert O.rel_errors(b, want)[0] < 1e-12
+
+
+def test_admission_waits_for_memory_and_rejects_what_never_fits():
+    N, S, n = 1024, 16, 4000
+    # room for one 16 MiB grid with its request, not for two
+    with migrid.Server(hbm_budget=32 * 2**20, max_inflight=1) as srv:
+        parts = [inputs(i, N, S, n, seed=21) for i in range(2)]
+        t1 = srv.submit(1, *parts[0], N, S)
+        srv.wait(t1)
+        # job 2's grid does not fit beside job 1's: it waits until job 1 is dropped
+        t2 = srv.submit(2, *parts[1], N, S)
+        done, _ = srv.poll(t2)
+        assert not done
+        assert srv.stats()["queued"] == 1
+        srv.drop_job(1)
+        srv.wait(t2)
+        assert O.rel_errors(srv.read_grid(2), ref(*parts[1], N, S))[0] < 1e-5
+        big = inputs(0, 2048, S, 100, seed=1)  # a 64 MiB grid
+        t3 = srv.submit(3, *big, 2048, S)
+        with pytest.raises(migrid.MigridError) as e:
+            srv.wait(t3)
+        assert e.value.code == -5
+        assert srv.stats()["failed"] == 1
+
+
+def test_bad_requests_fail_alone():
+    N, S = 128, 8
+    uv, vals, gcf = inputs(0, N, S, 500, seed=2)
+    with migrid.Server() as srv:
+        srv.wait(srv.submit(1, uv, vals, gcf, N, S))
+        t = srv.submit(1, uv, vals, gcf, 2 * N, S)  # job 1 holds an N x N grid
+        with pytest.raises(migrid.MigridError):
+            srv.wait(t)
+        with pytest.raises(migrid.MigridError):
+            srv.submit(1, uv, vals, gcf, N, 7)  # odd support: refused at submit
+        srv.wait(srv.submit(1, uv, vals, gcf, N, S))
+        assert O.rel_errors(srv.read_grid(1), 2 * ref(uv, vals, gcf, N, S))[0] < 1e-5
