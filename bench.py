@@ -329,6 +329,27 @@ def main_serve(args, torch, migrid):
     print(json.dumps(out), flush=True)
 
 
+def main_copy(args, torch):
+    """Plain device-to-device copy of 1 GiB: the calibration kernel for the
+    HBM byte counters (it reads and writes exactly 2^30 bytes per launch)."""
+    n = 2**28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    print(json.dumps({"metric": "device copy GB/s (read + write)", "value": 2 * 4 * n / ms / 1e6,
+                      "unit": "GB/s", "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n,
+                      "ms_per_launch": ms, "steps": args.steps}), flush=True)
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(p):
@@ -353,9 +374,10 @@ def main():
     ap.add_argument("--stage-min", type=int, default=0, help="tiled: staging threshold")
     ap.add_argument("--wg-per-cu", type=int, default=0, help="tiled: workgroups per CU")
     ap.add_argument("--dist", default="clustered", choices=["clustered", "uniform"])
-    ap.add_argument("--mode", default="grid", choices=["grid", "batch", "serve"],
+    ap.add_argument("--mode", default="grid", choices=["grid", "batch", "serve", "copy"],
                     help="grid: one 8192^2 job (default); batch: many jobs in one persistent "
-                         "launch; serve: Poisson request stream through the serving loop")
+                         "launch; serve: Poisson request stream through the serving loop; "
+                         "copy: 1 GiB device copy (byte-counter calibration)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--jobs", type=int, default=64, help="batch/serve: job ids")
     ap.add_argument("--large-jobs", type=int, default=4, help="serve: 8192^2 job ids")
@@ -378,6 +400,8 @@ def main():
         return main_batch(args, torch, migrid)
     if args.mode == "serve":
         return main_serve(args, torch, migrid)
+    if args.mode == "copy":
+        return main_copy(args, torch)
     N, S, npts = args.N, args.S, args.npts
     t0 = time.perf_counter()
     uv, vals = gen_clustered(npts, N, S) if args.dist == "clustered" else gen_uniform(npts, N, S)
