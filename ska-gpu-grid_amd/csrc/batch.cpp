@@ -39,8 +39,6 @@ struct migrid_batch {
 
 namespace migrid {
 
-static size_t ebytes(int t) { return t == MIGRID_F64 ? 8 : 4; }
-
 // Grid arguments of one job (same checks as a single call).
 int job_args(const migrid_job* j, const migrid_opts& o, GridArgs& a)
 {
@@ -75,13 +73,8 @@ int job_args(const migrid_job* j, const migrid_opts& o, GridArgs& a)
 int run_plan(const TiledPlan& p, const std::vector<GridArgs>& args, const migrid_opts& o,
              char* arena, void* bins, Profiler* prof, hipStream_t s)
 {
-    if (!o.accumulate)
-        for (const GridArgs& a : args) {
-            const size_t gb = (size_t)a.N * a.N * 2 * ebytes(a.out_type);
-            MG_TRY(launch_zero(a.grid, gb, s));
-            if (a.grid_lo) MG_TRY(launch_zero(a.grid_lo, gb, s));
-        }
-    MG_TRY(tiled_prepare(p, arena, bins, s));
+    (void)args;
+    MG_TRY(tiled_prepare(p, arena, bins, !o.accumulate, s));
     if (prof) MG_TRY(prof->begin(s));
     MG_TRY(tiled_execute(p, arena, s));
     if (prof) MG_TRY(prof->end(s));

@@ -16,15 +16,36 @@ struct BinGeom {
 };
 
 BinGeom bin_geom(int64_t N, int64_t S, int H);
-size_t bin_scratch_bytes(int64_t npts, const BinGeom& g);
 
-// perm[0..count): stable key order of binned visibility indices;
-// offsets[0..nkeys]: CSR starts, offsets[nkeys] = count.  Asynchronous.
-int bin_sort(const BinGeom& g, int in_type, const void* uv, int64_t npts, void* scratch,
-             int32_t* perm, int32_t* offsets, hipStream_t s);
+// One job of a multi-job sort (device-resident table).  Its elements are
+// [e_base, e_base + npts) of the concatenated arrays; its keys are
+// [k_base, k_base + nkeys], the last one the sentinel of visibilities that
+// touch no pixel.
+struct BinJob {
+    const void* uv;
+    int64_t npts;
+    int64_t e_base;
+    int64_t k_base;
+    int N, S, H, nb;
+};
+// Writes n host job descriptions to device memory (kernel launches; graph-safe).
+int put_binjobs(BinJob* dst, const BinJob* src, int n, hipStream_t s);
+
+// Scratch bytes for E elements over K keys.
+size_t bin_scratch_bytes(int64_t E, int64_t K);
+
+// Stable sort of the E elements of njobs jobs by key (K keys in all, K <=
+// 2^31): perm[0..E) the element indices in key order (each job's binned
+// visibilities, then its sentinels, job after job); offsets[0..K) the CSR
+// starts, offsets[k] = number of elements with key < k -- for job j,
+// offsets[k_base + nkeys] ends its binned visibilities.  Asynchronous.
+int bin_sort(const BinJob* jobs, int njobs, int64_t max_npts, int64_t E, int64_t K, int in_type,
+             void* scratch, int32_t* perm, int32_t* offsets, hipStream_t s);
 
 int64_t scan_scratch_elems(int64_t n);
 int exclusive_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* partial, bool write_total,
                    hipStream_t s);
+// x[i] = min(x[i..n)), in place.
+int suffix_min(int32_t* x, int64_t n, int32_t* partial, hipStream_t s);
 
 }  // namespace migrid

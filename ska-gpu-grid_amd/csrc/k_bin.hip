@@ -1,24 +1,33 @@
-// Bin sort of visibilities by (sub-kernel k, bin_y, bin_x).
+// Bin sort of visibilities by (job, sub-kernel k, bin_y, bin_x).
 //
 // Replaces the reference's host qsort (grid.cu:467-475), vis2ints
 // (grid_gpu.cu:352-365) and set_bookmarks (:366-390) with an on-device,
-// stable sort:
+// stable sort over the visibilities of any number of jobs at once (their
+// elements concatenated, each job owning a range of keys):
 //   1. k_keys     per visibility: q = floor(uv) + S/2, k = 8*sub_y + sub_x,
-//                 key = (k*nb + q_y/H)*nb + q_x/H (or the sentinel nkeys for a
-//                 visibility that touches no pixel) and a key histogram;
-//   2. LSD radix sort of (key, index) in 8-bit digits; each pass ranks a block's
-//      2048 elements stably with wave64 ballots (a lane's peers are the lanes
-//      whose digit matches in all 8 bit-ballots; its rank is the popcount of
-//      the peers below it, __builtin_amdgcn_mbcnt) and an LDS prefix over the
-//      block's waves;
-//   3. CSR offsets = exclusive scan of the key histogram (the bookmarks).
+//                 key = job base + (k*nb + q_y/H)*nb + q_x/H, or the job's
+//                 sentinel (base + nkeys) for a visibility that touches no
+//                 pixel; one block row per job (blockIdx.y);
+//   2. LSD radix sort of (key, element) in 8-bit digits; each pass ranks a
+//      block's 2048 elements stably with wave64 ballots (a lane's peers are
+//      the lanes whose digit matches in all 8 bit-ballots; its rank is the
+//      popcount of the peers below it, __builtin_amdgcn_mbcnt) and an LDS
+//      prefix over the block's waves; the first pass takes the element index
+//      as its value;
+//   3. CSR offsets (the bookmarks) from the sorted keys: the first position
+//      of every present key is marked, and a suffix minimum fills the keys
+//      that are absent -- offsets[k] = number of elements with key < k,
+//      without a histogram of atomics.
 // The permutation is the stable order, so it is bit-identical to a serial
 // counting sort of the same keys.
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "bin.hpp"
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "table.hpp"
 
 namespace migrid {
 namespace {
@@ -41,14 +50,16 @@ __device__ __forceinline__ int mbcnt(uint64_t m)
 }
 
 template <typename TI>
-__global__ __launch_bounds__(256) void k_keys(const TI* __restrict__ uv, int64_t npts, int N, int S,
-                                              int H, int nb, uint32_t* __restrict__ keys,
-                                              uint32_t* __restrict__ idx,
-                                              int32_t* __restrict__ hist)
+__global__ __launch_bounds__(256) void k_keys(const BinJob* __restrict__ jobs,
+                                              uint32_t* __restrict__ keys)
 {
+    const BinJob jb = jobs[blockIdx.y];
+    const TI* __restrict__ uv = (const TI*)jb.uv;
+    const int N = jb.N, S = jb.S, H = jb.H, nb = jb.nb;
     const uint32_t nkeys = (uint32_t)kSubKernels * nb * nb;
+    const uint32_t kbase = (uint32_t)jb.k_base;
     const int qmax = N + S - 2;
-    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < npts;
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < jb.npts;
          n += (int64_t)gridDim.x * blockDim.x) {
         const TI u = uv[2 * n], v = uv[2 * n + 1];
         const TI fu = floor_t(u), fv = floor_t(v);
@@ -61,10 +72,20 @@ __global__ __launch_bounds__(256) void k_keys(const TI* __restrict__ uv, int64_t
         if (qxf >= 0 && qyf >= 0 && qxf <= (TI)qmax && qyf <= (TI)qmax) {
             const int qx = (int)qxf, qy = (int)qyf;
             key = ((uint32_t)(kOversample * sy + sx) * nb + qy / H) * nb + qx / H;
-            atomicAdd(&hist[key], 1);
         }
-        keys[n] = key;
-        idx[n] = (uint32_t)n;
+        keys[jb.e_base + n] = kbase + key;
+    }
+}
+
+// First position of every present key: marks[key] = p where the sorted key
+// changes (marks starts filled with the element count).
+__global__ __launch_bounds__(256) void k_marks(const uint32_t* __restrict__ sk, int64_t n,
+                                               int32_t* __restrict__ marks)
+{
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = sk[p];
+        if (p == 0 || sk[p - 1] != k) marks[k] = (int32_t)p;
     }
 }
 
@@ -109,7 +130,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
         int d = 0;
         if (valid) {
             key = kin[i];
-            val = vin[i];
+            val = vin ? vin[i] : (uint32_t)i;  // the first pass: the element index
             d = (key >> shift) & 255;
         }
         // peers: active lanes with the same digit
@@ -218,9 +239,108 @@ __global__ __launch_bounds__(256) void k_scan_final(const int32_t* __restrict__ 
     }
 }
 
+// ---- suffix minimum of int32 (in place) ------------------------------------
+// x[i] = min(x[i], x[i+1], ..., x[n-1]).
+__device__ __forceinline__ int wave_suffix_min(int x)
+{
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_down(x, o, 64);
+        if (lane + o < 64) x = min(x, y);
+    }
+    return x;
+}
+
+__global__ __launch_bounds__(256) void k_smin_reduce(const int32_t* __restrict__ x, int64_t n,
+                                                     int32_t* __restrict__ partial)
+{
+    __shared__ int ws[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    int m = INT_MAX;
+    for (int j = 0; j < kScanItems; j++) {
+        const int64_t i = base + j * 256 + threadIdx.x;
+        if (i < n) m = min(m, x[i]);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_down(m, o, 64));
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = min(min(ws[0], ws[1]), min(ws[2], ws[3]));
+}
+
+// Single block: partial[b] = min over the blocks after b (exclusive).
+__global__ __launch_bounds__(1024) void k_smin_partials(int32_t* __restrict__ p, int64_t n)
+{
+    __shared__ int ws[16];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = INT_MAX;
+    __syncthreads();
+    for (int64_t b = n - 1; b >= 0; b -= 1024) {
+        // chunk [b - 1023, b], thread t takes element b - t (descending)
+        const int64_t i = b - threadIdx.x;
+        const int x = i >= 0 ? p[i] : INT_MAX;
+        // inclusive prefix min in thread order = suffix min in element order
+        const int lane = threadIdx.x & 63;
+        int inc = x;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc = min(inc, y);
+        }
+        if (lane == 63) ws[threadIdx.x >> 6] = inc;
+        __syncthreads();
+        int wmin = INT_MAX;
+        for (int w = 0; w < (int)(threadIdx.x >> 6); w++) wmin = min(wmin, ws[w]);
+        const int c = carry;
+        const int prev = __shfl_up(inc, 1, 64);  // all lanes take part
+        const int excl_in_chunk = min(wmin, lane ? prev : INT_MAX);
+        if (i >= 0) p[i] = min(c, excl_in_chunk);
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = min(c, min(wmin, inc));
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_smin_final(int32_t* __restrict__ x, int64_t n,
+                                                    const int32_t* __restrict__ partial)
+{
+    __shared__ int ws[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int m = INT_MAX;
+    for (int j = kScanItems - 1; j >= 0; j--) {
+        const int64_t i = base + j;
+        v[j] = i < n ? x[i] : INT_MAX;
+        m = min(m, v[j]);
+    }
+    // suffix min over the threads after this one, then the blocks after this one
+    const int inc = wave_suffix_min(m);
+    const int lane = threadIdx.x & 63;
+    if (lane == 0) ws[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    const int later_in_wave = __shfl_down(inc, 1, 64);
+    int run = lane < 63 ? later_in_wave : INT_MAX;
+    for (int w = (int)(threadIdx.x >> 6) + 1; w < 4; w++) run = min(run, ws[w]);
+    run = min(run, partial[blockIdx.x]);
+    for (int j = kScanItems - 1; j >= 0; j--) {
+        const int64_t i = base + j;
+        run = min(run, v[j]);
+        if (i < n) x[i] = run;
+    }
+}
+
 }  // namespace
 
 int64_t scan_scratch_elems(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+int suffix_min(int32_t* x, int64_t n, int32_t* partial, hipStream_t s)
+{
+    if (n <= 0) return MIGRID_OK;
+    const int64_t nblk = (n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_smin_reduce, dim3(nblk), dim3(256), 0, s, x, n, partial);
+    hipLaunchKernelGGL(k_smin_partials, dim3(1), dim3(1024), 0, s, partial, nblk);
+    hipLaunchKernelGGL(k_smin_final, dim3(nblk), dim3(256), 0, s, x, n, partial);
+    MG_HIP(hipGetLastError());
+    return MIGRID_OK;
+}
 
 int exclusive_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* partial, bool write_total,
                    hipStream_t s)
@@ -249,72 +369,82 @@ BinGeom bin_geom(int64_t N, int64_t S, int H)
     return g;
 }
 
-size_t bin_scratch_bytes(int64_t npts, const BinGeom& g)
+namespace {
+size_t align64(size_t b) { return (b + 63) & ~(size_t)63; }
+}  // namespace
+
+int put_binjobs(BinJob* dst, const BinJob* src, int n, hipStream_t s)
 {
-    const int64_t nblk = (npts + kRsTile - 1) / kRsTile;
-    size_t b = 0;
-    b += 4 * (size_t)npts * 4;                         // keys/idx double buffers
-    b += (size_t)(g.nkeys + 1) * 4;                    // key histogram
-    b += (size_t)256 * (nblk + 1) * 4 * 2;             // digit counts + offsets
-    const int64_t big = g.nkeys + 1 > 256 * (nblk + 1) ? g.nkeys + 1 : 256 * (nblk + 1);
-    b += (size_t)scan_scratch_elems(big) * 4;          // scan partials
-    return b + 64 * 5;
+    return put_table(dst, src, n, s);
 }
 
-int bin_sort(const BinGeom& g, int in_type, const void* uv, int64_t npts, void* scratch,
-             int32_t* perm, int32_t* offsets, hipStream_t s)
+size_t bin_scratch_bytes(int64_t E, int64_t K)
 {
-    const int64_t nblk = (npts + kRsTile - 1) / kRsTile;
+    const int64_t nblk = (E + kRsTile - 1) / kRsTile;
+    size_t b = 4 * align64((size_t)(E > 0 ? E : 1) * 4);  // keys/values double buffers
+    b += 2 * align64((size_t)256 * (nblk + 1) * 4);        // digit counts + offsets
+    const int64_t big = K > 256 * (nblk + 1) ? K : 256 * (nblk + 1);
+    b += align64((size_t)scan_scratch_elems(big) * 4);     // scan partials
+    return b;
+}
+
+int bin_sort(const BinJob* jobs, int njobs, int64_t max_npts, int64_t E, int64_t K, int in_type,
+             void* scratch, int32_t* perm, int32_t* offsets, hipStream_t s)
+{
+    const int64_t nblk = (E + kRsTile - 1) / kRsTile;
     char* p = (char*)scratch;
     auto take = [&](size_t bytes) {
         char* r = p;
-        p += (bytes + 63) & ~(size_t)63;
+        p += align64(bytes);
         return r;
     };
-    uint32_t* k0 = (uint32_t*)take((size_t)npts * 4);
-    uint32_t* v0 = (uint32_t*)take((size_t)npts * 4);
-    uint32_t* k1 = (uint32_t*)take((size_t)npts * 4);
-    uint32_t* v1 = (uint32_t*)take((size_t)npts * 4);
-    int32_t* hist = (int32_t*)take((size_t)(g.nkeys + 1) * 4);
+    const size_t eb = (size_t)(E > 0 ? E : 1) * 4;
+    uint32_t* k0 = (uint32_t*)take(eb);
+    uint32_t* v0 = (uint32_t*)take(eb);
+    uint32_t* k1 = (uint32_t*)take(eb);
+    uint32_t* v1 = (uint32_t*)take(eb);
     int32_t* counts = (int32_t*)take((size_t)256 * (nblk + 1) * 4);
     int32_t* coffs = (int32_t*)take((size_t)256 * (nblk + 1) * 4);
-    const int64_t big = g.nkeys + 1 > 256 * (nblk + 1) ? g.nkeys + 1 : 256 * (nblk + 1);
+    const int64_t big = K > 256 * (nblk + 1) ? K : 256 * (nblk + 1);
     int32_t* partial = (int32_t*)take((size_t)scan_scratch_elems(big) * 4);
 
-    MG_TRY(fill_u32(hist, 0u, g.nkeys + 1, s));
-    if (npts > 0) {
-        int blocks = (int)((npts + 255) / 256 < 4096 ? (npts + 255) / 256 : 4096);
+    // offsets start as the marks array: every key "first seen at E"
+    MG_TRY(fill_u32(offsets, (uint32_t)E, K, s));
+    if (E > 0) {
+        const int64_t per_job = (max_npts + 255) / 256;
+        const int64_t cap = njobs < 16384 ? 16384 / njobs : 1;
+        const int bx = (int)(per_job < 1 ? 1 : per_job < cap ? per_job : cap);
         if (in_type == MIGRID_F64)
-            hipLaunchKernelGGL(k_keys<double>, dim3(blocks), dim3(256), 0, s, (const double*)uv,
-                               npts, g.N, g.S, g.H, g.nb, k0, v0, hist);
+            hipLaunchKernelGGL(k_keys<double>, dim3(bx, njobs), dim3(256), 0, s, jobs, k0);
         else
-            hipLaunchKernelGGL(k_keys<float>, dim3(blocks), dim3(256), 0, s, (const float*)uv, npts,
-                               g.N, g.S, g.H, g.nb, k0, v0, hist);
+            hipLaunchKernelGGL(k_keys<float>, dim3(bx, njobs), dim3(256), 0, s, jobs, k0);
         MG_HIP(hipGetLastError());
-        // number of key bits (the sentinel nkeys included)
         int bits = 1;
-        while (((int64_t)1 << bits) <= g.nkeys) bits++;
-        int passes = (bits + 7) / 8;
-        uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
+        while (((int64_t)1 << bits) < K) bits++;  // keys are < K
+        const int passes = (bits + 7) / 8;
+        uint32_t *ki = k0, *vi = nullptr, *ko = k1, *vo = v1;
+        uint32_t* vspare = v0;
         for (int pass = 0; pass < passes; pass++) {
             const int shift = 8 * pass;
-            if (pass == passes - 1) vo = (uint32_t*)perm;  // last pass writes the permutation
-            hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(kRsThreads), 0, s, ki, npts, shift,
-                               counts);
+            if (pass == passes - 1) vo = (uint32_t*)perm;  // the last pass writes the permutation
+            hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(kRsThreads), 0, s, ki, E, shift, counts);
             MG_TRY(exclusive_scan(counts, 256 * nblk, coffs, partial, false, s));
-            hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(kRsThreads), 0, s, ki, vi, ko, vo,
-                               npts, shift, coffs);
+            hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(kRsThreads), 0, s, ki, vi, ko, vo, E,
+                               shift, coffs);
             MG_HIP(hipGetLastError());
             uint32_t* tk = ki;
             ki = ko;
             ko = tk;
-            uint32_t* tv = vi;
+            uint32_t* next_vo = vi ? vi : vspare;
             vi = vo;
-            vo = tv;
+            vo = next_vo;
         }
-        // perm: sorted indices of binned visibilities first (sentinels sort last)
+        // ki: the sorted keys
+        const int mb = (int)((E + 255) / 256 < 8192 ? (E + 255) / 256 : 8192);
+        hipLaunchKernelGGL(k_marks, dim3(mb), dim3(256), 0, s, ki, E, offsets);
+        MG_HIP(hipGetLastError());
+        MG_TRY(suffix_min(offsets, K, partial, s));
     }
-    MG_TRY(exclusive_scan(hist, g.nkeys, offsets, partial, true, s));
     return MIGRID_OK;
 }
 
@@ -340,11 +470,24 @@ extern "C" int migrid_ctx_bin_sort(migrid_ctx* ctx, int32_t in_type, const void*
         return fail(MIGRID_EINVAL, "in_type=%d", in_type);
     if (!perm || !offsets || (npts > 0 && !uv)) return fail(MIGRID_EINVAL, "NULL buffer");
     const BinGeom g = bin_geom(img_dim, gcf_dim, bin);
-    if (g.nkeys >= (int64_t)1 << 31) return fail(MIGRID_EINVAL, "too many bins (%lld)", (long long)g.nkeys);
+    if (g.nkeys >= (int64_t)1 << 31 || npts >= (int64_t)1 << 31)
+        return fail(MIGRID_EINVAL, "too many bins (%lld) or visibilities", (long long)g.nkeys);
     MG_HIP(hipSetDevice(ctx->device));
     hipStream_t s = (hipStream_t)stream;
-    MG_TRY(ctx->bins.ensure(bin_scratch_bytes(npts, g)));
-    MG_TRY(bin_sort(g, in_type, uv, npts, ctx->bins.p, perm, offsets, s));
+    const int64_t K = g.nkeys + 1;  // the job's keys and its sentinel
+    MG_TRY(ctx->bins.ensure(bin_scratch_bytes(npts, K) + 256));
+    BinJob jb;
+    jb.uv = uv;
+    jb.npts = npts;
+    jb.e_base = 0;
+    jb.k_base = 0;
+    jb.N = g.N;
+    jb.S = g.S;
+    jb.H = g.H;
+    jb.nb = g.nb;
+    BinJob* djob = (BinJob*)ctx->bins.p;
+    MG_TRY(put_binjobs(djob, &jb, 1, s));
+    MG_TRY(bin_sort(djob, 1, npts, npts, K, in_type, (char*)ctx->bins.p + 256, perm, offsets, s));
     int32_t total = 0;
     MG_HIP(hipMemcpyAsync(&total, offsets + g.nkeys, 4, hipMemcpyDeviceToHost, s));
     MG_HIP(hipStreamSynchronize(s));

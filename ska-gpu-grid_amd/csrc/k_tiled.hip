@@ -41,6 +41,7 @@
 #include "bin.hpp"
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "table.hpp"
 #include "tiled.hpp"
 
 namespace migrid {
@@ -109,13 +110,18 @@ constexpr int kValCompatS = 1;  // value x COMPUTE_GCF tap value, scatter form
 constexpr int kValCompatI = 2;  // the same with the fixed-point dx of gather/window
 constexpr int kValDebug = 3;    // (1, n): the DEBUG1 coverage map
 
+// Records of every job in one launch (blockIdx.y = job): the job's binned
+// visibilities are the sorted positions [offs[0], offs[nkeys]).
 template <typename TI, typename R, int VT>
-__global__ __launch_bounds__(256) void k_pack(const TI* __restrict__ uv, const TI* __restrict__ vals,
-                                              const int32_t* __restrict__ perm,
-                                              const int32_t* __restrict__ count_p, int S, int pitch,
-                                              const TI* __restrict__ gcf, R* __restrict__ rec)
+__global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
+                                              const int32_t* __restrict__ perm, R* __restrict__ rec)
 {
-    const int64_t count = *count_p;
+    const PackJob jb = jobs[blockIdx.y];
+    const TI* __restrict__ uv = (const TI*)jb.uv;
+    const TI* __restrict__ vals = (const TI*)jb.vals;
+    const TI* __restrict__ gcf = (const TI*)jb.gcf;
+    const int S = jb.S, pitch = jb.pitch;
+    const int64_t p_lo = jb.offs[0], p_hi = jb.offs[jb.nkeys];
     float p1 = 0.f, p2 = 0.f;
     if (VT == kValCompatS || VT == kValCompatI) {
         const int64_t e = (int64_t)S * (S - 1) / 2 - 1;  // grid_gpu.cu:104-107
@@ -123,9 +129,9 @@ __global__ __launch_bounds__(256) void k_pack(const TI* __restrict__ uv, const T
         p1 = (float)(2 * 3.1415926 * w);
         p2 = (float)(p1 * T);
     }
-    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < count;
+    for (int64_t j = p_lo + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < p_hi;
          j += (int64_t)gridDim.x * blockDim.x) {
-        const int n = perm[j];
+        const int64_t n = perm[j] - jb.e_base;
         const TI u = uv[2 * n], v = uv[2 * n + 1];
         const TI fu = floor_t(u), fv = floor_t(v);
         const int mx = (int)fu, my = (int)fv;
@@ -155,7 +161,7 @@ __global__ __launch_bounds__(256) void k_pack(const TI* __restrict__ uv, const T
             vi = ni;
         } else if (VT == kValDebug) {
             vr = (TI)1;
-            vi = (TI)n;
+            vi = (TI)n;  // the visibility's index within its job
         }
         R r;
         r.q = qx | (qy << 16);
@@ -170,11 +176,15 @@ __global__ __launch_bounds__(256) void k_pack(const TI* __restrict__ uv, const T
 // One wave per tile; lane = sub-kernel k.  Counts the tile's visibilities per
 // k from the CSR offsets and cuts k into items of about `target` visibilities
 // (target <= 0: one item per tile).
-__global__ __launch_bounds__(256) void k_items(const int32_t* __restrict__ offs, TileGeom g,
-                                               int64_t target, int job, Item* __restrict__ items,
-                                               QueueState* __restrict__ qs,
-                                               uint8_t* __restrict__ split)
+// Every job in one launch: blockIdx.y = job.
+__global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs, int64_t target,
+                                               Item* __restrict__ items,
+                                               QueueState* __restrict__ qs)
 {
+    const int job = blockIdx.y;
+    const TileGeom g = jobs[job].g;
+    const int32_t* __restrict__ offs = jobs[job].offs;
+    uint8_t* __restrict__ split = (uint8_t*)jobs[job].split;
     const int lane = threadIdx.x & 63;
     const int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t ntiles = (int64_t)g.ntx * g.ntx;
@@ -591,39 +601,35 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
     }
 }
 
-// Writes one job descriptor (passed by value, so the launch owns a copy).
-__global__ void k_set_desc(JobDesc* dst, JobDesc d) { *dst = d; }
-
 // The GCF table in the staged type (FP32 or BF16 pairs), converted once per
 // call so staging and the direct path move 8- or 4-byte elements.
 template <typename TI, typename G>
-__global__ __launch_bounds__(256) void k_convert_gcf(const TI* __restrict__ src, G* __restrict__ dst,
-                                                     int64_t n)
+__global__ __launch_bounds__(256) void k_convert_gcf(const ConvJob* __restrict__ jobs)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+    const ConvJob c = jobs[blockIdx.y];
+    const TI* __restrict__ src = (const TI*)c.src;
+    G* __restrict__ dst = (G*)c.dst;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < c.n;
          i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = Stage<G>::make((double)src[2 * i], (double)src[2 * i + 1]);
 }
 
-int launch_convert(const TiledJob& t, int stage, char* arena, hipStream_t s)
+int launch_convert(const ConvJob* table, int n, int64_t max_elems, int stage, bool in64,
+                   hipStream_t s)
 {
-    const int64_t n = (int64_t)kSubKernels * t.a.S * t.a.S;
-    const int blocks = (int)std::min<int64_t>(2048, (n + 255) / 256);
-    void* dst = arena + t.off_gtab;
-    const bool in64 = t.a.in_type == MIGRID_F64;
+    if (n <= 0) return MIGRID_OK;
+    const int64_t per = (max_elems + 255) / 256;
+    const int bx = (int)(per < 256 ? (per < 1 ? 1 : per) : 256);
+    const dim3 grid(bx, n);
     if (stage == kStageBf16) {
         if (in64)
-            hipLaunchKernelGGL((k_convert_gcf<double, Bf16x2>), dim3(blocks), dim3(256), 0, s,
-                               (const double*)t.a.gcf, (Bf16x2*)dst, n);
+            hipLaunchKernelGGL((k_convert_gcf<double, Bf16x2>), grid, dim3(256), 0, s, table);
         else
-            hipLaunchKernelGGL((k_convert_gcf<float, Bf16x2>), dim3(blocks), dim3(256), 0, s,
-                               (const float*)t.a.gcf, (Bf16x2*)dst, n);
+            hipLaunchKernelGGL((k_convert_gcf<float, Bf16x2>), grid, dim3(256), 0, s, table);
     } else if (stage == kStageF32) {
-        hipLaunchKernelGGL((k_convert_gcf<double, float2>), dim3(blocks), dim3(256), 0, s,
-                           (const double*)t.a.gcf, (float2*)dst, n);
+        hipLaunchKernelGGL((k_convert_gcf<double, float2>), grid, dim3(256), 0, s, table);
     } else {
-        hipLaunchKernelGGL((k_convert_gcf<float, double2>), dim3(blocks), dim3(256), 0, s,
-                           (const float*)t.a.gcf, (double2*)dst, n);
+        hipLaunchKernelGGL((k_convert_gcf<float, double2>), grid, dim3(256), 0, s, table);
     }
     MG_HIP(hipGetLastError());
     return MIGRID_OK;
@@ -683,13 +689,14 @@ int run_modes(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
 }
 
 template <typename TI, typename R>
-void launch_pack(const GridArgs& a, int vt, const int32_t* perm, const int32_t* count, R* rec,
+void launch_pack(const PackJob* table, int n, int64_t max_npts, int vt, const int32_t* perm, R* rec,
                  hipStream_t s)
 {
-    const int blocks = (int)std::min<int64_t>(4096, (a.npts + 255) / 256 + 1);
-#define MG_PACK(VT)                                                                          \
-    hipLaunchKernelGGL((k_pack<TI, R, VT>), dim3(blocks), dim3(256), 0, s, (const TI*)a.uv,  \
-                       (const TI*)a.vals, perm, count, a.S, stage_pitch(a.S), (const TI*)a.gcf, rec)
+    const int64_t per = (max_npts + 255) / 256;
+    const int64_t cap = n < 8192 ? 8192 / n : 1;
+    const int bx = (int)(per < 1 ? 1 : per < cap ? per : cap);
+#define MG_PACK(VT)                                                                            \
+    hipLaunchKernelGGL((k_pack<TI, R, VT>), dim3(bx, n), dim3(256), 0, s, table, perm, rec)
     switch (vt) {
     case kValCompatS: MG_PACK(kValCompatS); break;
     case kValCompatI: MG_PACK(kValCompatI); break;
@@ -737,6 +744,7 @@ TileCfg pick_tile(int S, int tile_opt)
 int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
 {
     if (n < 1) return fail(MIGRID_EINVAL, "tiled plan with %d jobs", n);
+    if (n > 65535) return fail(MIGRID_ENOTSUP, "a tiled plan takes at most 65535 jobs (%d)", n);
     const bool f32arith = o.arith == MIGRID_ARITH_F32 || o.arith == MIGRID_ARITH_F32_BF16GCF;
     if (o.strategy != MIGRID_TILED && o.arith != MIGRID_ARITH_NATIVE)
         return fail(MIGRID_ENOTSUP, "gather/window run native arithmetic only");
@@ -767,14 +775,21 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
                                                           : 0;
         const TileCfg tc = pick_tile(x.S, o.tile);
         k.T = tc.T;
-        if (j == 0)
+        int vt = kValPlain;
+        if (x.debug)
+            vt = kValDebug;
+        else if (x.gcf_mode == MIGRID_GCF_COMPAT)
+            vt = o.strategy == MIGRID_TILED ? kValCompatS : kValCompatI;
+        if (j == 0) {
             p.key = k;
-        else if (!(k == p.key))
-            return fail(MIGRID_ENOTSUP, "batched jobs must share types, layout, GCF mode and tile");
+            p.vt = vt;
+        } else if (!(k == p.key) || vt != p.vt) {
+            return fail(MIGRID_ENOTSUP,
+                        "batched jobs must share types, layout, GCF mode, coverage mode and tile");
+        }
         TiledJob& t = p.jobs[j];
         t.a = x;
         t.bg = bin_geom(x.N, x.S, tc.H);
-        if (t.bg.nkeys >= ((int64_t)1 << 31)) return fail(MIGRID_ENOTSUP, "tiled: too many bins");
         t.g.N = x.N;
         t.g.S = x.S;
         t.g.T = tc.T;
@@ -782,18 +797,13 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         t.g.nb = t.bg.nb;
         t.g.ntx = (x.N + tc.T - 1) / tc.T;
         t.rec64 = k.stage == kStageF64;
-        t.vt = kValPlain;
-        if (x.debug)
-            t.vt = kValDebug;
-        else if (x.gcf_mode == MIGRID_GCF_COMPAT)
-            t.vt = o.strategy == MIGRID_TILED ? kValCompatS : kValCompatI;
+        t.vt = vt;
+        t.e_base = p.E;
+        t.k_base = p.K;
+        p.E += x.npts;
+        p.K += t.bg.nkeys + 1;  // the job's keys and its sentinel
+        p.max_npts = std::max<int64_t>(p.max_npts, x.npts);
         const int64_t ntiles = (int64_t)t.g.ntx * t.g.ntx;
-        t.off_rec = off;
-        off += align256((size_t)x.npts * (t.rec64 ? sizeof(RecF64) : sizeof(RecF32)));
-        t.off_perm = off;
-        off += align256((size_t)(x.npts > 0 ? x.npts : 1) * 4);
-        t.off_offs = off;
-        off += align256(((size_t)t.bg.nkeys + 1) * 4);
         t.off_split = off;
         off += align256((size_t)ntiles);
         // a table in the staged type unless the caller's table already is one
@@ -803,11 +813,32 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
             const size_t gsz = k.stage == kStageBf16 ? 4 : k.stage == kStageF32 ? 8 : 16;
             t.off_gtab = off;
             off += align256((size_t)kSubKernels * x.S * x.S * gsz);
+            p.nconv++;
+            p.max_gcf = std::max<int64_t>(p.max_gcf, (int64_t)kSubKernels * x.S * x.S);
         }
+        const int64_t gb = (int64_t)x.N * x.N * 2 * (x.out_type == MIGRID_F64 ? 8 : 4);
+        p.max_grid = std::max(p.max_grid, gb);
         item_cap += ntiles * kSubKernels;
-        p.bin_bytes = std::max(p.bin_bytes, bin_scratch_bytes(x.npts, t.bg));
         p.smax = std::max(p.smax, x.S);
     }
+    if (p.K >= ((int64_t)1 << 31) || p.E >= ((int64_t)1 << 31))
+        return fail(MIGRID_ENOTSUP, "tiled: %lld keys / %lld visibilities in one plan (limit 2^31)",
+                    (long long)p.K, (long long)p.E);
+    const bool rec64 = p.key.stage == kStageF64;
+    p.off_perm = off;
+    off += align256((size_t)(p.E > 0 ? p.E : 1) * 4);
+    p.off_offs = off;
+    off += align256((size_t)p.K * 4);
+    p.off_rec = off;
+    off += align256((size_t)(p.E > 0 ? p.E : 1) * (rec64 ? sizeof(RecF64) : sizeof(RecF32)));
+    p.off_bjobs = off;
+    off += align256(sizeof(BinJob) * n);
+    p.off_pjobs = off;
+    off += align256(sizeof(PackJob) * n);
+    p.off_cjobs = off;
+    off += align256(sizeof(ConvJob) * n);
+    p.off_zjobs = off;
+    off += align256(sizeof(ZeroJob) * 2 * n);
     p.off_desc = off;
     off += align256(sizeof(JobDesc) * n);
     p.off_items = off;
@@ -815,6 +846,7 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.off_qs = off;
     off += align256(sizeof(QueueState));
     p.arena_bytes = off;
+    p.bin_bytes = bin_scratch_bytes(p.E, p.K);
     p.item_cap = item_cap;
     p.target = o.strategy == MIGRID_TILED ? (o.vis_per_wave > 0 ? o.vis_per_wave : 4096) : 0;
     p.stage_min = o.stage_min;
@@ -830,8 +862,8 @@ void tiled_descs(const TiledPlan& p, char* arena, std::vector<JobDesc>& d)
     for (size_t j = 0; j < p.jobs.size(); j++) {
         const TiledJob& t = p.jobs[j];
         d[j].g = t.g;
-        d[j].rec = arena + t.off_rec;
-        d[j].offs = (const int32_t*)(arena + t.off_offs);
+        d[j].rec = arena + p.off_rec;  // CSR offsets are positions in the shared records
+        d[j].offs = (const int32_t*)(arena + p.off_offs) + t.k_base;
         d[j].split = (const uint8_t*)(arena + t.off_split);
         d[j].gcf = t.convert ? (const void*)(arena + t.off_gtab) : t.a.gcf;
         d[j].grid = t.a.grid;
@@ -844,42 +876,92 @@ void tiled_descs(const TiledPlan& p, char* arena, std::vector<JobDesc>& d)
 int tiled_write_descs(const TiledPlan& p, char* arena, const std::vector<JobDesc>& d,
                       hipStream_t s)
 {
-    JobDesc* dst = (JobDesc*)(arena + p.off_desc);
-    for (size_t j = 0; j < d.size(); j++)
-        hipLaunchKernelGGL(k_set_desc, dim3(1), dim3(1), 0, s, dst + j, d[j]);
-    MG_HIP(hipGetLastError());
-    return MIGRID_OK;
+    return put_table((JobDesc*)(arena + p.off_desc), d.data(), (int)d.size(), s);
 }
 
-int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_t s)
+int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, bool zero_grids,
+                  hipStream_t s)
 {
+    const int n = (int)p.jobs.size();
     QueueState* qs = (QueueState*)(arena + p.off_qs);
     Item* items = (Item*)(arena + p.off_items);
+    int32_t* perm = (int32_t*)(arena + p.off_perm);
+    int32_t* offs = (int32_t*)(arena + p.off_offs);
     MG_TRY(fill_u32(qs, 0u, sizeof(QueueState) / 4, s));
-    for (size_t j = 0; j < p.jobs.size(); j++) {
+    // job tables
+    std::vector<BinJob> bj(n);
+    std::vector<PackJob> pj(n);
+    std::vector<ConvJob> cj;
+    std::vector<ZeroJob> zj;
+    for (int j = 0; j < n; j++) {
         const TiledJob& t = p.jobs[j];
-        const GridArgs& a = t.a;
-        int32_t* perm = (int32_t*)(arena + t.off_perm);
-        int32_t* offs = (int32_t*)(arena + t.off_offs);
-        if (t.convert) MG_TRY(launch_convert(t, p.key.stage, arena, s));
-        MG_TRY(bin_sort(t.bg, a.in_type, a.uv, a.npts, bin_scratch, perm, offs, s));
-        const int32_t* count = offs + t.bg.nkeys;
-        if (a.npts > 0) {
-            if (a.in_type == MIGRID_F64) {
-                if (t.rec64)
-                    launch_pack<double, RecF64>(a, t.vt, perm, count, (RecF64*)(arena + t.off_rec), s);
-                else
-                    launch_pack<double, RecF32>(a, t.vt, perm, count, (RecF32*)(arena + t.off_rec), s);
-            } else {
-                launch_pack<float, RecF32>(a, t.vt, perm, count, (RecF32*)(arena + t.off_rec), s);
-            }
-            MG_HIP(hipGetLastError());
+        bj[j].uv = t.a.uv;
+        bj[j].npts = t.a.npts;
+        bj[j].e_base = t.e_base;
+        bj[j].k_base = t.k_base;
+        bj[j].N = t.bg.N;
+        bj[j].S = t.bg.S;
+        bj[j].H = t.bg.H;
+        bj[j].nb = t.bg.nb;
+        pj[j].uv = t.a.uv;
+        pj[j].vals = t.a.vals;
+        pj[j].gcf = t.a.gcf;
+        pj[j].offs = offs + t.k_base;
+        pj[j].e_base = t.e_base;
+        pj[j].nkeys = t.bg.nkeys;
+        pj[j].S = t.a.S;
+        pj[j].pitch = stage_pitch(t.a.S);
+        if (t.convert) {
+            ConvJob c;
+            c.src = t.a.gcf;
+            c.dst = arena + t.off_gtab;
+            c.n = (int64_t)kSubKernels * t.a.S * t.a.S;
+            cj.push_back(c);
         }
-        const int64_t ntiles = (int64_t)t.g.ntx * t.g.ntx;
-        hipLaunchKernelGGL(k_items, dim3((unsigned)((ntiles + 3) / 4)), dim3(256), 0, s, offs, t.g,
-                           p.target, (int)j, items, qs, (uint8_t*)(arena + t.off_split));
+        if (zero_grids) {
+            ZeroJob z;
+            z.bytes = (int64_t)t.a.N * t.a.N * 2 * (t.a.out_type == MIGRID_F64 ? 8 : 4);
+            z.p = t.a.grid;
+            zj.push_back(z);
+            if (t.a.grid_lo) {
+                z.p = t.a.grid_lo;
+                zj.push_back(z);
+            }
+        }
+    }
+    BinJob* dbj = (BinJob*)(arena + p.off_bjobs);
+    PackJob* dpj = (PackJob*)(arena + p.off_pjobs);
+    MG_TRY(put_table(dbj, bj.data(), n, s));
+    MG_TRY(put_table(dpj, pj.data(), n, s));
+    if (!zj.empty()) {
+        ZeroJob* dzj = (ZeroJob*)(arena + p.off_zjobs);
+        MG_TRY(put_table(dzj, zj.data(), (int)zj.size(), s));
+        MG_TRY(zero_multi(dzj, (int)zj.size(), p.max_grid, s));
+    }
+    if (!cj.empty()) {
+        ConvJob* dcj = (ConvJob*)(arena + p.off_cjobs);
+        MG_TRY(put_table(dcj, cj.data(), (int)cj.size(), s));
+        MG_TRY(launch_convert(dcj, (int)cj.size(), p.max_gcf, p.key.stage,
+                              p.key.in_type == MIGRID_F64, s));
+    }
+    MG_TRY(bin_sort(dbj, n, p.max_npts, p.E, p.K, p.key.in_type, bin_scratch, perm, offs, s));
+    if (p.E > 0) {
+        void* rec = arena + p.off_rec;
+        if (p.key.in_type == MIGRID_F64) {
+            if (p.key.stage == kStageF64)
+                launch_pack<double, RecF64>(dpj, n, p.max_npts, p.vt, perm, (RecF64*)rec, s);
+            else
+                launch_pack<double, RecF32>(dpj, n, p.max_npts, p.vt, perm, (RecF32*)rec, s);
+        } else {
+            launch_pack<float, RecF32>(dpj, n, p.max_npts, p.vt, perm, (RecF32*)rec, s);
+        }
         MG_HIP(hipGetLastError());
     }
+    int64_t max_tiles = 0;
+    for (const TiledJob& t : p.jobs) max_tiles = std::max<int64_t>(max_tiles, (int64_t)t.g.ntx * t.g.ntx);
+    hipLaunchKernelGGL(k_items, dim3((unsigned)((max_tiles + 3) / 4), n), dim3(256), 0, s,
+                       (const JobDesc*)(arena + p.off_desc), p.target, items, qs);
+    MG_HIP(hipGetLastError());
     return MIGRID_OK;
 }
 
@@ -908,7 +990,7 @@ int launch_tiled(migrid_ctx* ctx, const GridArgs& a, const migrid_opts& o, hipSt
     std::vector<JobDesc> d;
     tiled_descs(p, arena, d);
     MG_TRY(tiled_write_descs(p, arena, d, s));
-    MG_TRY(tiled_prepare(p, arena, ctx->bins.p, s));
+    MG_TRY(tiled_prepare(p, arena, ctx->bins.p, false, s));
     MG_TRY(ctx->prof.begin(s));
     MG_TRY(tiled_execute(p, arena, s));
     MG_TRY(ctx->prof.end(s));

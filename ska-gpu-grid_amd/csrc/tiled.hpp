@@ -92,18 +92,51 @@ struct TiledJob {
     TileGeom g;
     bool rec64 = false;
     int vt = 0;
-    size_t off_rec = 0, off_perm = 0, off_offs = 0, off_split = 0;  // in the arena
+    int64_t e_base = 0;  // first element in the concatenated visibilities
+    int64_t k_base = 0;  // first key (its keys are [k_base, k_base + nkeys])
+    size_t off_split = 0;  // in the arena
     bool convert = false;  // the GCF table is converted to the staged type
     size_t off_gtab = 0;
 };
 
-// Device layout of one owner launch over n jobs: per-job records, bin-sort
-// output and split flags, then the job table, the item queue and its state.
+// Record packing of one job (device table, blockIdx.y of k_pack): the job's
+// binned visibilities are the sorted positions [offs[0], offs[nkeys]).
+struct PackJob {
+    const void* uv;
+    const void* vals;
+    const void* gcf;
+    const int32_t* offs;
+    int64_t e_base;
+    int64_t nkeys;
+    int S, pitch;
+};
+
+// GCF table conversion to the staged type (device table).
+struct ConvJob {
+    const void* src;
+    void* dst;
+    int64_t n;  // complex elements
+};
+
+// Device layout of one owner launch over n jobs.  The jobs' visibilities
+// are sorted together (one key space, each job a range of it): the arena
+// holds the permutation, the CSR offsets and the records of all jobs, the
+// per-job split flags and converted GCF tables, the job tables, the item
+// queue and its state.
 struct TiledPlan {
     KernelKey key;
     std::vector<TiledJob> jobs;
+    int64_t E = 0;         // visibilities of all jobs
+    int64_t K = 0;         // keys of all jobs
+    int64_t max_npts = 0;  // the largest job
+    int vt = 0;            // record value transform (shared)
+    size_t off_perm = 0, off_offs = 0, off_rec = 0;
+    size_t off_bjobs = 0, off_pjobs = 0, off_cjobs = 0, off_zjobs = 0;
+    int nconv = 0;             // jobs whose GCF is converted
+    int64_t max_gcf = 0;       // largest GCF table (complex elements)
+    int64_t max_grid = 0;      // largest grid (bytes)
     size_t off_desc = 0, off_items = 0, off_qs = 0, arena_bytes = 0;
-    size_t bin_bytes = 0;  // bin-sort scratch (reused job after job)
+    size_t bin_bytes = 0;  // bin-sort scratch
     int64_t item_cap = 0;
     int64_t target = 0;  // visibilities per work item (0: one item per tile)
     int smax = 0;
@@ -127,8 +160,10 @@ void tiled_descs(const TiledPlan& p, char* arena, std::vector<JobDesc>& d);
 // Writes the job table into the arena with kernels (graph-safe).
 int tiled_write_descs(const TiledPlan& p, char* arena, const std::vector<JobDesc>& d,
                       hipStream_t s);
-// Queue reset, then per job: bin sort, record packing, work items.
-int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_t s);
+// Queue reset, GCF conversion, one bin sort of all jobs, record packing and
+// work items (one launch each), and, with zero_grids, the grid clears.
+int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, bool zero_grids,
+                  hipStream_t s);
 // The persistent owner kernel over all queued items.
 int tiled_execute(const TiledPlan& p, char* arena, hipStream_t s);
 int dispatch_owner(const KernelKey& k, const OwnerLaunch& L, hipStream_t s);
