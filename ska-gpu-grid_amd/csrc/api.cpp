@@ -156,17 +156,18 @@ static int grid_device(migrid_ctx* ctx, const migrid_job* job, const migrid_opts
     a.sigma = o.gcf_sigma > 0 ? o.gcf_sigma : a.S / 6.0;
     a.beta = o.gcf_beta;
 
-    if (!o.accumulate) {
-        size_t gbytes = (size_t)a.N * a.N * 2 * elem_bytes(a.out_type);
-        MG_TRY(launch_zero(a.grid, gbytes, s));
-        if (a.grid_lo) MG_TRY(launch_zero(a.grid_lo, gbytes, s));
-    }
     if (o.strategy == MIGRID_SCATTER) {
+        if (!o.accumulate) {
+            size_t gbytes = (size_t)a.N * a.N * 2 * elem_bytes(a.out_type);
+            MG_TRY(launch_zero(a.grid, gbytes, s));
+            if (a.grid_lo) MG_TRY(launch_zero(a.grid_lo, gbytes, s));
+        }
         if (ctx) MG_TRY(ctx->prof.begin(s));
         MG_TRY(launch_scatter(a, s));
         if (ctx) MG_TRY(ctx->prof.end(s));
         return MIGRID_OK;
     }
+    // the tiled strategies clear a fresh grid themselves (tiles nobody stores)
     return launch_tiled(ctx, a, o, s);
 }
 

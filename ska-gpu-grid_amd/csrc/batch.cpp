@@ -74,7 +74,8 @@ int run_plan(const TiledPlan& p, const std::vector<GridArgs>& args, const migrid
              char* arena, void* bins, Profiler* prof, hipStream_t s)
 {
     (void)args;
-    MG_TRY(tiled_prepare(p, arena, bins, !o.accumulate, s));
+    (void)o;  // fresh grids: the plan's own passes clear what the owner kernel does not store
+    MG_TRY(tiled_prepare(p, arena, bins, s));
     if (prof) MG_TRY(prof->begin(s));
     MG_TRY(tiled_execute(p, arena, s));
     if (prof) MG_TRY(prof->end(s));

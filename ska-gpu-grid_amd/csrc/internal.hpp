@@ -62,13 +62,6 @@ int validate_shape(int64_t npts, int64_t N, int64_t S);
 // ---- launchers (each returns MIGRID_OK or a recorded error) ------------
 int launch_scatter(const GridArgs& a, hipStream_t s);
 int launch_zero(void* p, size_t bytes, hipStream_t s);  // fill kernel, graph-safe
-// Zero fills of n buffers (a device table; 4-byte aligned, sizes multiples
-// of 4 bytes) in one launch.
-struct ZeroJob {
-    void* p;
-    int64_t bytes;
-};
-int zero_multi(const ZeroJob* table, int n, int64_t max_bytes, hipStream_t s);
 int fill_u32(void* p, uint32_t v, int64_t n, hipStream_t s);
 
 // Device scratch owned by a context.

@@ -172,33 +172,76 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
     }
 }
 
+// ---- per-tile counts ----------------------------------------------------------------
+// tcount[tile * 64 + k] = visibilities of sub-kernel k whose window reaches the
+// tile: the tile's bin rows summed from the CSR offsets.  Block (ty, k), one
+// thread per tile column, so a bin row's offsets are read by neighbouring
+// threads; blockIdx.y = job.
+__global__ __launch_bounds__(128) void k_tile_counts(const JobDesc* __restrict__ jobs)
+{
+    const JobDesc& d = jobs[blockIdx.y];
+    const TileGeom g = d.g;
+    const int32_t* __restrict__ offs = d.offs;
+    int32_t* __restrict__ tcount = (int32_t*)d.tcount;
+    for (int64_t b = blockIdx.x; b < (int64_t)g.ntx * kSubKernels; b += gridDim.x) {
+        const int ty = (int)(b / kSubKernels), k = (int)(b % kSubKernels);
+        const int byl = ty * g.T / g.H;
+        const int byh = min(g.nb - 1, (ty * g.T + g.T + g.S - 2) / g.H);
+        for (int tx = threadIdx.x; tx < g.ntx; tx += blockDim.x) {
+            const int bxl = tx * g.T / g.H;
+            const int bxh = min(g.nb - 1, (tx * g.T + g.T + g.S - 2) / g.H);
+            int cnt = 0;
+            for (int by = byl; by <= byh; by++) {
+                const int64_t key = ((int64_t)k * g.nb + by) * g.nb;
+                cnt += offs[key + bxh + 1] - offs[key + bxl];
+            }
+            tcount[((int64_t)ty * g.ntx + tx) * kSubKernels + k] = cnt;
+        }
+    }
+}
+
+// Zeroes the pixels of one tile (the whole wave; lanes along a row).
+template <typename TO>
+__device__ void zero_tile(TO* grid, int N, int X0, int Y0, int T, bool planar)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t NN = (int64_t)N * N;
+    const int xe = min(N, X0 + T), ye = min(N, Y0 + T);
+    for (int y = Y0; y < ye; y++)
+        for (int x = X0 + lane; x < xe; x += 64) {
+            const int64_t p = (int64_t)y * N + x;
+            if (planar) {
+                grid[p] = (TO)0;
+                grid[NN + p] = (TO)0;
+            } else {
+                grid[2 * p] = (TO)0;
+                grid[2 * p + 1] = (TO)0;
+            }
+        }
+}
+
 // ---- work items -----------------------------------------------------------------
-// One wave per tile; lane = sub-kernel k.  Counts the tile's visibilities per
-// k from the CSR offsets and cuts k into items of about `target` visibilities
-// (target <= 0: one item per tile).
-// Every job in one launch: blockIdx.y = job.
+// One wave per tile; lane = sub-kernel k.  Cuts the tile's sub-kernels into
+// items of about `target` visibilities (target <= 0: one item per tile).
+// Every job in one launch: blockIdx.y = job.  For a fresh grid the wave
+// zeroes its tile when the owner kernel will not store it whole: no items
+// (nothing stores it) or several (they add to it atomically).
+template <typename TO>
 __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs, int64_t target,
-                                               Item* __restrict__ items,
+                                               int planar, Item* __restrict__ items,
                                                QueueState* __restrict__ qs)
 {
     const int job = blockIdx.y;
     const TileGeom g = jobs[job].g;
-    const int32_t* __restrict__ offs = jobs[job].offs;
+    const int32_t* __restrict__ tcount = jobs[job].tcount;
     uint8_t* __restrict__ split = (uint8_t*)jobs[job].split;
+    const bool fresh = jobs[job].fresh != 0;
     const int lane = threadIdx.x & 63;
     const int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t ntiles = (int64_t)g.ntx * g.ntx;
     if (tile >= ntiles) return;
     const int tx = (int)(tile % g.ntx), ty = (int)(tile / g.ntx);
-    const int bxl = tx * g.T / g.H, byl = ty * g.T / g.H;
-    const int bxh = min(g.nb - 1, (tx * g.T + g.T + g.S - 2) / g.H);
-    const int byh = min(g.nb - 1, (ty * g.T + g.T + g.S - 2) / g.H);
-    const int k = lane;
-    int64_t cnt = 0;
-    for (int by = byl; by <= byh; by++) {
-        const int64_t key = ((int64_t)k * g.nb + by) * g.nb;
-        cnt += offs[key + bxh + 1] - offs[key + bxl];
-    }
+    const int64_t cnt = tcount[tile * kSubKernels + lane];
     // inclusive scan over the 64 lanes
     int64_t inc = cnt;
     for (int o = 1; o < 64; o <<= 1) {
@@ -206,7 +249,14 @@ __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs,
         if (lane >= o) inc += y;
     }
     const int64_t total = __shfl(inc, 63, 64);
-    if (total == 0) return;
+    if (total == 0) {
+        if (fresh) {
+            zero_tile((TO*)jobs[job].grid, g.N, tx * g.T, ty * g.T, g.T, planar != 0);
+            if (jobs[job].grid_lo)
+                zero_tile((TO*)jobs[job].grid_lo, g.N, tx * g.T, ty * g.T, g.T, planar != 0);
+        }
+        return;
+    }
     int nit = 1;
     if (target > 0) {
         int64_t want = (total + target - 1) / target;
@@ -239,13 +289,30 @@ __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs,
         items[slot] = it;
     }
     if (lane == 0) split[tile] = n_items > 1;
+    if (fresh && n_items > 1) {
+        zero_tile((TO*)jobs[job].grid, g.N, tx * g.T, ty * g.T, g.T, planar != 0);
+        if (jobs[job].grid_lo)
+            zero_tile((TO*)jobs[job].grid_lo, g.N, tx * g.T, ty * g.T, g.T, planar != 0);
+    }
 }
 
 // ---- flush helpers --------------------------------------------------------------
+// atomic: the tile is split over several items (adds with hardware atomics);
+// otherwise the tile is this item's alone: stored outright when the grid is
+// fresh, else read-add-written.
 template <typename TO, typename TA>
 __device__ __forceinline__ void flush_px(TO* grid, TO* lo, int64_t idx_re, int64_t idx_im, TA re,
-                                         TA im, bool atomic)
+                                         TA im, bool atomic, bool fresh)
 {
+    if (!atomic && fresh) {
+        grid[idx_re] = (TO)re;
+        grid[idx_im] = (TO)im;
+        if (lo != nullptr) {  // two-sum onto zero leaves no error
+            lo[idx_re] = (TO)0;
+            lo[idx_im] = (TO)0;
+        }
+        return;
+    }
     if (lo != nullptr) {
         // FP32 grid with a two-sum compensation grid (Knuth's TwoSum on the
         // value the hardware added to): grid + lo carries the rounding error
@@ -396,12 +463,12 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
         const int job = __builtin_amdgcn_readfirstlane(it.job);
         const TileGeom g = jobs[job].g;
         ConstPtr<R> rec = (ConstPtr<R>)(uintptr_t)jobs[job].rec;
-        ConstPtr<int32_t> offs = (ConstPtr<int32_t>)(uintptr_t)jobs[job].offs;
         const G* __restrict__ gtab = (const G*)jobs[job].gcf;  // table in the staged type
         TO* __restrict__ grid = (TO*)jobs[job].grid;
         TO* __restrict__ grid_lo = (TO*)jobs[job].grid_lo;
         const double sigma = jobs[job].sigma, beta = jobs[job].beta;
         const bool atomic = jobs[job].split[tile] != 0;
+        const bool fresh = jobs[job].fresh != 0;
         const int S = g.S;
         const int B = stage_border(S);
         const int pitch = stage_pitch(S);
@@ -428,8 +495,6 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
         // this wave's bins (q in [Xw, Xw + S + 14]) and the workgroup's
         const int bxl = Xw / g.H, bxh = min(g.nb - 1, (Xw + S + 14) / g.H);
         const int byl = Yw / g.H, byh = min(g.nb - 1, (Yw + S + 14) / g.H);
-        const int Bxl = X0 / g.H, Bxh = min(g.nb - 1, (X0 + g.T + S - 2) / g.H);
-        const int Byl = Y0 / g.H, Byh = min(g.nb - 1, (Y0 + g.T + S - 2) / g.H);
         const int lane_off = ly * pitch + lx;
 
         Cplx<TA> acc[2][2];
@@ -442,12 +507,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
         // ballots give the non-empty and the staged sub-kernels of this item
         uint64_t nz_mask, st_mask;
         {
-            int cnt = 0;
-            if (lane >= klo && lane <= khi)
-                for (int by = Byl; by <= Byh; by++) {
-                    const int64_t key = ((int64_t)lane * g.nb + by) * g.nb;
-                    cnt += offs[key + Bxh + 1] - offs[key + Bxl];
-                }
+            const int cnt =
+                (lane >= klo && lane <= khi) ? jobs[job].tcount[(int64_t)tile * kSubKernels + lane] : 0;
             nz_mask = __ballot(cnt > 0);
             st_mask = GM != 0 ? nz_mask : __ballot(cnt >= stage_min && cnt > 0);
         }
@@ -592,9 +653,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
                 if (x >= g.N) continue;
                 const int64_t p = (int64_t)y * g.N + x;
                 if (PLANAR)
-                    flush_px(grid, grid_lo, p, NN + p, acc[i][j].re, acc[i][j].im, atomic);
+                    flush_px(grid, grid_lo, p, NN + p, acc[i][j].re, acc[i][j].im, atomic, fresh);
                 else
-                    flush_px(grid, grid_lo, 2 * p, 2 * p + 1, acc[i][j].re, acc[i][j].im, atomic);
+                    flush_px(grid, grid_lo, 2 * p, 2 * p + 1, acc[i][j].re, acc[i][j].im, atomic,
+                             fresh);
             }
         }
         __syncthreads();  // s_item reuse
@@ -806,6 +868,8 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         const int64_t ntiles = (int64_t)t.g.ntx * t.g.ntx;
         t.off_split = off;
         off += align256((size_t)ntiles);
+        t.off_tcnt = off;
+        off += align256((size_t)ntiles * kSubKernels * 4);
         // a table in the staged type unless the caller's table already is one
         t.convert = k.gm == 0 && !((k.stage == kStageF64 && x.in_type == MIGRID_F64) ||
                                    (k.stage == kStageF32 && x.in_type == MIGRID_F32));
@@ -816,8 +880,6 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
             p.nconv++;
             p.max_gcf = std::max<int64_t>(p.max_gcf, (int64_t)kSubKernels * x.S * x.S);
         }
-        const int64_t gb = (int64_t)x.N * x.N * 2 * (x.out_type == MIGRID_F64 ? 8 : 4);
-        p.max_grid = std::max(p.max_grid, gb);
         item_cap += ntiles * kSubKernels;
         p.smax = std::max(p.smax, x.S);
     }
@@ -837,8 +899,6 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     off += align256(sizeof(PackJob) * n);
     p.off_cjobs = off;
     off += align256(sizeof(ConvJob) * n);
-    p.off_zjobs = off;
-    off += align256(sizeof(ZeroJob) * 2 * n);
     p.off_desc = off;
     off += align256(sizeof(JobDesc) * n);
     p.off_items = off;
@@ -851,6 +911,7 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.target = o.strategy == MIGRID_TILED ? (o.vis_per_wave > 0 ? o.vis_per_wave : 4096) : 0;
     p.stage_min = o.stage_min;
     p.wg_per_cu = o.wg_per_cu;
+    p.fresh = !o.accumulate;
     const size_t gsz = p.key.stage == kStageBf16 ? 4 : p.key.stage == kStageF32 ? 8 : 16;
     p.lds = (size_t)stage_rows(p.smax) * stage_pitch(p.smax) * gsz;
     return MIGRID_OK;
@@ -864,12 +925,15 @@ void tiled_descs(const TiledPlan& p, char* arena, std::vector<JobDesc>& d)
         d[j].g = t.g;
         d[j].rec = arena + p.off_rec;  // CSR offsets are positions in the shared records
         d[j].offs = (const int32_t*)(arena + p.off_offs) + t.k_base;
+        d[j].tcount = (const int32_t*)(arena + t.off_tcnt);
         d[j].split = (const uint8_t*)(arena + t.off_split);
         d[j].gcf = t.convert ? (const void*)(arena + t.off_gtab) : t.a.gcf;
         d[j].grid = t.a.grid;
         d[j].grid_lo = t.a.grid_lo;
         d[j].sigma = t.a.sigma;
         d[j].beta = t.a.beta;
+        d[j].fresh = p.fresh ? 1 : 0;
+        d[j].pad = 0;
     }
 }
 
@@ -879,8 +943,7 @@ int tiled_write_descs(const TiledPlan& p, char* arena, const std::vector<JobDesc
     return put_table((JobDesc*)(arena + p.off_desc), d.data(), (int)d.size(), s);
 }
 
-int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, bool zero_grids,
-                  hipStream_t s)
+int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_t s)
 {
     const int n = (int)p.jobs.size();
     QueueState* qs = (QueueState*)(arena + p.off_qs);
@@ -892,7 +955,6 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, bool zero_
     std::vector<BinJob> bj(n);
     std::vector<PackJob> pj(n);
     std::vector<ConvJob> cj;
-    std::vector<ZeroJob> zj;
     for (int j = 0; j < n; j++) {
         const TiledJob& t = p.jobs[j];
         bj[j].uv = t.a.uv;
@@ -918,26 +980,11 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, bool zero_
             c.n = (int64_t)kSubKernels * t.a.S * t.a.S;
             cj.push_back(c);
         }
-        if (zero_grids) {
-            ZeroJob z;
-            z.bytes = (int64_t)t.a.N * t.a.N * 2 * (t.a.out_type == MIGRID_F64 ? 8 : 4);
-            z.p = t.a.grid;
-            zj.push_back(z);
-            if (t.a.grid_lo) {
-                z.p = t.a.grid_lo;
-                zj.push_back(z);
-            }
-        }
     }
     BinJob* dbj = (BinJob*)(arena + p.off_bjobs);
     PackJob* dpj = (PackJob*)(arena + p.off_pjobs);
     MG_TRY(put_table(dbj, bj.data(), n, s));
     MG_TRY(put_table(dpj, pj.data(), n, s));
-    if (!zj.empty()) {
-        ZeroJob* dzj = (ZeroJob*)(arena + p.off_zjobs);
-        MG_TRY(put_table(dzj, zj.data(), (int)zj.size(), s));
-        MG_TRY(zero_multi(dzj, (int)zj.size(), p.max_grid, s));
-    }
     if (!cj.empty()) {
         ConvJob* dcj = (ConvJob*)(arena + p.off_cjobs);
         MG_TRY(put_table(dcj, cj.data(), (int)cj.size(), s));
@@ -957,10 +1004,21 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, bool zero_
         }
         MG_HIP(hipGetLastError());
     }
-    int64_t max_tiles = 0;
-    for (const TiledJob& t : p.jobs) max_tiles = std::max<int64_t>(max_tiles, (int64_t)t.g.ntx * t.g.ntx);
-    hipLaunchKernelGGL(k_items, dim3((unsigned)((max_tiles + 3) / 4), n), dim3(256), 0, s,
-                       (const JobDesc*)(arena + p.off_desc), p.target, items, qs);
+    int64_t max_tiles = 0, max_ntx = 0;
+    for (const TiledJob& t : p.jobs) {
+        max_tiles = std::max<int64_t>(max_tiles, (int64_t)t.g.ntx * t.g.ntx);
+        max_ntx = std::max<int64_t>(max_ntx, t.g.ntx);
+    }
+    const JobDesc* ddesc = (const JobDesc*)(arena + p.off_desc);
+    const int cb = (int)std::min<int64_t>(max_ntx * kSubKernels, n < 8 ? 8192 : 65536 / n);
+    hipLaunchKernelGGL(k_tile_counts, dim3(cb, n), dim3(128), 0, s, ddesc);
+    const dim3 ig((unsigned)((max_tiles + 3) / 4), n);
+    if (p.key.out_type == MIGRID_F64)
+        hipLaunchKernelGGL(k_items<double>, ig, dim3(256), 0, s, ddesc, p.target,
+                           p.key.planar ? 1 : 0, items, qs);
+    else
+        hipLaunchKernelGGL(k_items<float>, ig, dim3(256), 0, s, ddesc, p.target,
+                           p.key.planar ? 1 : 0, items, qs);
     MG_HIP(hipGetLastError());
     return MIGRID_OK;
 }
@@ -983,14 +1041,20 @@ int launch_tiled(migrid_ctx* ctx, const GridArgs& a, const migrid_opts& o, hipSt
     if (!ctx) return fail(MIGRID_EINVAL, "tiled strategies need a context");
     TiledPlan p;
     MG_TRY(tiled_plan(&a, 1, o, p));
-    if (a.npts == 0) return MIGRID_OK;
+    if (a.npts == 0) {  // nothing to grid: a fresh grid is only cleared
+        if (!p.fresh) return MIGRID_OK;
+        const size_t gb = (size_t)a.N * a.N * 2 * (a.out_type == MIGRID_F64 ? 8 : 4);
+        MG_TRY(launch_zero(a.grid, gb, s));
+        if (a.grid_lo) MG_TRY(launch_zero(a.grid_lo, gb, s));
+        return MIGRID_OK;
+    }
     MG_TRY(ctx->bins.ensure(p.bin_bytes));
     MG_TRY(ctx->records.ensure(p.arena_bytes));
     char* arena = (char*)ctx->records.p;
     std::vector<JobDesc> d;
     tiled_descs(p, arena, d);
     MG_TRY(tiled_write_descs(p, arena, d, s));
-    MG_TRY(tiled_prepare(p, arena, ctx->bins.p, false, s));
+    MG_TRY(tiled_prepare(p, arena, ctx->bins.p, s));
     MG_TRY(ctx->prof.begin(s));
     MG_TRY(tiled_execute(p, arena, s));
     MG_TRY(ctx->prof.end(s));

@@ -22,20 +22,6 @@ __global__ __launch_bounds__(256) void k_fill32(uint32_t* __restrict__ p, uint32
         p[i] = v;
 }
 
-// One buffer per block row (blockIdx.y): 16-byte stores, then the 4-byte tail.
-__global__ __launch_bounds__(256) void k_zero_multi(const ZeroJob* __restrict__ t)
-{
-    const ZeroJob z = t[blockIdx.y];
-    // 16-byte stores when the buffer is 16-byte aligned, else 4-byte stores
-    const int64_t n16 = ((uintptr_t)z.p & 15) ? 0 : z.bytes / 16, n4 = z.bytes / 4;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    uint4* p16 = (uint4*)z.p;
-    for (int64_t i = i0; i < n16; i += stride) p16[i] = make_uint4(0, 0, 0, 0);
-    uint32_t* p4 = (uint32_t*)z.p;
-    for (int64_t i = 4 * n16 + i0; i < n4; i += stride) p4[i] = 0u;
-}
-
 int grid_blocks(int64_t n)
 {
     int64_t b = (n + 255) / 256;
@@ -49,17 +35,6 @@ int fill_u32(void* p, uint32_t v, int64_t n, hipStream_t s)
 {
     if (n <= 0) return MIGRID_OK;
     hipLaunchKernelGGL(k_fill32, dim3(grid_blocks(n)), dim3(256), 0, s, (uint32_t*)p, v, n);
-    MG_HIP(hipGetLastError());
-    return MIGRID_OK;
-}
-
-int zero_multi(const ZeroJob* table, int n, int64_t max_bytes, hipStream_t s)
-{
-    if (n <= 0 || max_bytes <= 0) return MIGRID_OK;
-    const int64_t per = (max_bytes / 16 + 255) / 256;
-    const int64_t cap = n < 8192 ? 8192 / n : 1;
-    const int bx = (int)(per < 1 ? 1 : per < cap ? per : cap);
-    hipLaunchKernelGGL(k_zero_multi, dim3(bx, n), dim3(256), 0, s, table);
     MG_HIP(hipGetLastError());
     return MIGRID_OK;
 }

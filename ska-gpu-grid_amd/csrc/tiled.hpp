@@ -54,11 +54,15 @@ struct JobDesc {
     TileGeom g;
     const void* rec;
     const int32_t* offs;
+    const int32_t* tcount;  // visibilities per (tile, k): tcount[tile * 64 + k]
     const uint8_t* split;
     const void* gcf;
     void* grid;
     void* grid_lo;
     double sigma, beta;
+    int32_t fresh;  // grid not cleared beforehand: unsplit tiles are stored,
+                    // empty and split tiles were zeroed by the work-item pass
+    int32_t pad;
 };
 
 // ---- host-side plan ---------------------------------------------------------
@@ -94,7 +98,7 @@ struct TiledJob {
     int vt = 0;
     int64_t e_base = 0;  // first element in the concatenated visibilities
     int64_t k_base = 0;  // first key (its keys are [k_base, k_base + nkeys])
-    size_t off_split = 0;  // in the arena
+    size_t off_split = 0, off_tcnt = 0;  // in the arena
     bool convert = false;  // the GCF table is converted to the staged type
     size_t off_gtab = 0;
 };
@@ -130,11 +134,11 @@ struct TiledPlan {
     int64_t K = 0;         // keys of all jobs
     int64_t max_npts = 0;  // the largest job
     int vt = 0;            // record value transform (shared)
+    bool fresh = false;    // fresh grids (cleared by the plan's own passes)
     size_t off_perm = 0, off_offs = 0, off_rec = 0;
-    size_t off_bjobs = 0, off_pjobs = 0, off_cjobs = 0, off_zjobs = 0;
+    size_t off_bjobs = 0, off_pjobs = 0, off_cjobs = 0;
     int nconv = 0;             // jobs whose GCF is converted
     int64_t max_gcf = 0;       // largest GCF table (complex elements)
-    int64_t max_grid = 0;      // largest grid (bytes)
     size_t off_desc = 0, off_items = 0, off_qs = 0, arena_bytes = 0;
     size_t bin_bytes = 0;  // bin-sort scratch
     int64_t item_cap = 0;
@@ -160,10 +164,10 @@ void tiled_descs(const TiledPlan& p, char* arena, std::vector<JobDesc>& d);
 // Writes the job table into the arena with kernels (graph-safe).
 int tiled_write_descs(const TiledPlan& p, char* arena, const std::vector<JobDesc>& d,
                       hipStream_t s);
-// Queue reset, GCF conversion, one bin sort of all jobs, record packing and
-// work items (one launch each), and, with zero_grids, the grid clears.
-int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, bool zero_grids,
-                  hipStream_t s);
+// Queue reset, GCF conversion, one bin sort of all jobs, record packing,
+// per-tile counts and work items (one launch each).  For fresh grids the
+// work-item pass zeroes the tiles the owner kernel will not store whole.
+int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_t s);
 // The persistent owner kernel over all queued items.
 int tiled_execute(const TiledPlan& p, char* arena, hipStream_t s);
 int dispatch_owner(const KernelKey& k, const OwnerLaunch& L, hipStream_t s);
