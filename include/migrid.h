@@ -219,6 +219,42 @@ int migrid_autotune(migrid_ctx* ctx, const migrid_job* jobs, int njobs, const mi
 /* Number of bins per axis for migrid_ctx_bin_sort. */
 int64_t migrid_bins_per_axis(int64_t img_dim, int64_t gcf_dim, int32_t bin);
 
+/* ---- input side: coordinates and per-visibility GCF index ------------------
+ * (u, v, w) in metres per row and channel frequencies in Hz -> grid pixel
+ * coordinates for the gridder, visibility n = row * nchan + chan (the order
+ * of the reference's HDF5 reader, hdf52struct.cpp:95-113):
+ *   u_pix = img_dim/2 + (u f / c) img_dim cell_rad, v_pix likewise,
+ * c = 299792458 m/s, cell_rad the image pixel size in radians.  With
+ * nplanes > 0 also a w-plane GCF index per visibility:
+ *   g = min(nplanes - 1, floor(|w f / c| / w_max * nplanes)).
+ * The reference's device convert() (grid_gpu.cu:45-51) is the normalised
+ * special case; its vis.gcfinx (vis.h:17) is an index of this kind, which no
+ * reference kernel reads.  Device pointers (uvw: nrow x 3 doubles, freq:
+ * nchan doubles, uv_out: nrow*nchan x 2 doubles); asynchronous on stream. */
+int migrid_uvw_to_pixels(const double* uvw, int64_t nrow, const double* freq, int32_t nchan,
+                         int64_t img_dim, double cell_rad, int32_t nplanes, double w_max,
+                         double* uv_out, int32_t* gcf_index_out, void* stream);
+
+/* Gridding with one GCF table per visibility: job->gcf holds ngcf tables of
+ * 64 * gcf_dim^2 complex values one after another, and gcf_index[n] in
+ * [0, ngcf) picks visibility n's table (device int32 array).  The
+ * visibilities are grouped by table on the device and gridded as one
+ * persistent launch of the owner kernel (strategy TILED, GATHER or WINDOW;
+ * SCATTER is taken as TILED), all groups flushing atomically into the one
+ * grid.  GCF mode TABLE only.  Synchronises with stream once (to read the
+ * group sizes); the gridding itself is asynchronous. */
+int migrid_ctx_grid_indexed(migrid_ctx* ctx, const migrid_job* job, int32_t ngcf,
+                            const int32_t* gcf_index, const migrid_opts* opts, void* stream);
+
+/* The reference's UVW text input (grid.cu:338-366): lines
+ * "%d,%d,%d: %f, %f, %f" (three integers, u, v, w), u and v scaled to the
+ * grid as the reference does, x = (double)(float)(u * (float)img_dim) / 2048.
+ * Host memory: reads up to max_rows lines into uv_out (max_rows x 2) and,
+ * when w_out is not NULL, w into w_out; *nrows_out = the rows read.  Blank
+ * lines are skipped; any other unreadable line is an error (EINVAL). */
+int migrid_read_uvw_text(const char* path, int64_t img_dim, int64_t max_rows, double* uv_out,
+                         double* w_out, int64_t* nrows_out);
+
 /* ---- serving loop (continuous batching) -----------------------------------
  * No counterpart in the reference, whose only entry is the blocking
  * per-job gridGPU call (grid_gpu.cu:675-819).  A server owns a worker thread

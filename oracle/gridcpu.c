@@ -491,3 +491,31 @@ void oracle_grid_const_tap(double* out, const double* uv, const double* vals, in
         }
     }
 }
+
+/* (u, v, w) in metres per row and channel frequencies in Hz -> grid pixel
+ * coordinates, visibility n = row * nchan + chan (the order of the
+ * reference's HDF5 reader: time outer, frequency inner, hdf52struct.cpp:95-113):
+ *   u_pix = N/2 + (u f / c) N cell, v_pix likewise, c = 299792458 m/s;
+ * with nplanes > 0 a w-plane GCF index
+ *   g = min(nplanes - 1, floor(|w f / c| / w_max * nplanes)).
+ * The reference's own normalised form, convert() (grid_gpu.cu:45-51), maps
+ * pin in [-1, 1) to (pin + 1) * asize and splits it into main and sub the way
+ * the gridder's index math does for non-negative coordinates. */
+void oracle_uvw_to_pixels(const double* uvw, int64_t nrow, const double* freq, int64_t nchan,
+                          int64_t N, double cell, int64_t nplanes, double w_max, double* uv,
+                          int32_t* gidx)
+{
+    const double c = 299792458.0;
+    for (int64_t r = 0; r < nrow; r++)
+        for (int64_t ch = 0; ch < nchan; ch++) {
+            const int64_t n = r * nchan + ch;
+            const double s = freq[ch] / c;
+            uv[2 * n] = 0.5 * (double)N + uvw[3 * r] * s * (double)N * cell;
+            uv[2 * n + 1] = 0.5 * (double)N + uvw[3 * r + 1] * s * (double)N * cell;
+            if (nplanes > 0 && gidx) {
+                const double wl = fabs(uvw[3 * r + 2] * s);
+                int64_t g = (int64_t)floor(wl / w_max * (double)nplanes);
+                gidx[n] = (int32_t)(g < nplanes - 1 ? g : nplanes - 1);
+            }
+        }
+}

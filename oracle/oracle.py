@@ -63,6 +63,8 @@ def lib():
         L.oracle_window_pixel.restype = _i64
         L.oracle_compute_gcf_phase.argtypes = [_dp, _i64, ctypes.c_double, ctypes.c_double, ctypes.c_int, _fp, _fp]
         L.oracle_grid_const_tap.argtypes = [_dp, _dp, _dp, _i64, _i64, _i64, _fp, _fp]
+        L.oracle_uvw_to_pixels.argtypes = [_dp, _i64, _dp, _i64, _i64, ctypes.c_double, _i64,
+                                           ctypes.c_double, _dp, _ip]
         _lib = L
     return _lib
 
@@ -244,6 +246,20 @@ def grid_const_tap(uv, vals, N, S, r2, i2):
     lib().oracle_grid_const_tap(_p(out, _dp), _p(uv, _dp), _p(vals, _dp), uv.shape[0], N, S,
                                 _p(r2, _fp), _p(i2, _fp))
     return out
+
+
+def uvw_to_pixels(uvw, freq, N, cell, nplanes=0, w_max=1.0):
+    """(nrow,3) metres, (nchan,) Hz -> (nrow*nchan, 2) pixels and, with
+    nplanes > 0, the w-plane index per visibility."""
+    uvw = _c(uvw, np.float64).reshape(-1, 3)
+    freq = _c(freq, np.float64).reshape(-1)
+    n = uvw.shape[0] * freq.shape[0]
+    uv = np.empty((n, 2), np.float64)
+    g = np.empty(n, np.int32) if nplanes > 0 else None
+    lib().oracle_uvw_to_pixels(_p(uvw, _dp), uvw.shape[0], _p(freq, _dp), freq.shape[0], N,
+                               float(cell), int(nplanes), float(w_max), _p(uv, _dp),
+                               _p(g, _ip) if g is not None else None)
+    return uv, g
 
 
 # ---- error metrics ----------------------------------------------------------

@@ -73,8 +73,19 @@ int job_args(const migrid_job* j, const migrid_opts& o, GridArgs& a)
 int run_plan(const TiledPlan& p, const std::vector<GridArgs>& args, const migrid_opts& o,
              char* arena, void* bins, Profiler* prof, hipStream_t s)
 {
-    (void)args;
-    (void)o;  // fresh grids: the plan's own passes clear what the owner kernel does not store
+    // fresh grids: the plan's own passes clear what the owner kernel does not
+    // store, except grids shared by several jobs, cleared here once each
+    if (p.shared && !o.accumulate) {
+        std::vector<void*> done;
+        for (const GridArgs& a : args) {
+            const size_t gb = (size_t)a.N * a.N * 2 * (a.out_type == MIGRID_F64 ? 8 : 4);
+            for (void* g : {a.grid, a.grid_lo}) {
+                if (!g || std::find(done.begin(), done.end(), g) != done.end()) continue;
+                MG_TRY(launch_zero(g, gb, s));
+                done.push_back(g);
+            }
+        }
+    }
     MG_TRY(tiled_prepare(p, arena, bins, s));
     if (prof) MG_TRY(prof->begin(s));
     MG_TRY(tiled_execute(p, arena, s));
@@ -276,3 +287,77 @@ int migrid_autotune(migrid_ctx* ctx, const migrid_job* jobs, int njobs, const mi
 }
 
 }  // extern "C"
+
+extern "C" int migrid_ctx_grid_indexed(migrid_ctx* ctx, const migrid_job* job, int32_t ngcf,
+                                       const int32_t* gcf_index, const migrid_opts* opts_in,
+                                       void* stream)
+{
+    clear_error();
+    if (!ctx || !job) return fail(MIGRID_EINVAL, "NULL argument");
+    migrid_opts o;
+    if (opts_in)
+        o = *opts_in;
+    else
+        migrid_opts_default(&o);
+    if (o.strategy == MIGRID_SCATTER) o.strategy = MIGRID_TILED;
+    if (o.gcf_mode != MIGRID_GCF_TABLE)
+        return fail(MIGRID_ENOTSUP, "indexed gridding reads GCF tables (gcf_mode TABLE)");
+    if (ngcf < 1 || ngcf >= (1 << 24)) return fail(MIGRID_EINVAL, "ngcf=%d", ngcf);
+    if (job->npts > 0 && !gcf_index) return fail(MIGRID_EINVAL, "gcf_index is NULL");
+    GridArgs whole;
+    MG_TRY(job_args(job, o, whole));
+    MG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = job->npts;
+    const size_t ce = job->in_type == MIGRID_F64 ? 16 : 8;  // complex element
+    const size_t gcf_elems = (size_t)kSubKernels * job->gcf_dim * job->gcf_dim;
+
+    // group the visibilities by index (device), then read the group sizes
+    const size_t vbytes = (size_t)(n > 0 ? n : 1) * ce;
+    const size_t misc = 2 * ((vbytes + 255) & ~(size_t)255) + (((size_t)(n > 0 ? n : 1) * 4 + 255) & ~(size_t)255) +
+                        ((size_t)(ngcf + 1) * 4 + 255);
+    MG_TRY(ctx->misc.ensure(misc));
+    char* m = (char*)ctx->misc.p;
+    void* guv = m;
+    void* gvals = m + ((vbytes + 255) & ~(size_t)255);
+    int32_t* perm = (int32_t*)(m + 2 * ((vbytes + 255) & ~(size_t)255));
+    int32_t* goffs = (int32_t*)((char*)perm + (((size_t)(n > 0 ? n : 1) * 4 + 255) & ~(size_t)255));
+    MG_TRY(ctx->bins.ensure(bin_scratch_bytes(n, ngcf + 1)));
+    MG_TRY(group_by_index(gcf_index, n, ngcf, job->in_type, job->uv, job->vals, guv, gvals,
+                          ctx->bins.p, perm, goffs, s));
+    std::vector<int32_t> h(ngcf + 1);
+    MG_HIP(hipMemcpyAsync(h.data(), goffs, (ngcf + 1) * 4, hipMemcpyDeviceToHost, s));
+    MG_HIP(hipStreamSynchronize(s));
+    if (h[ngcf] != n)
+        return fail(MIGRID_EINVAL, "%lld visibilities have a GCF index outside [0, %d)",
+                    (long long)(n - h[ngcf]), ngcf);
+
+    // one job per non-empty group, all into the caller's grid
+    std::vector<GridArgs> args;
+    for (int g = 0; g < ngcf; g++) {
+        const int64_t lo = h[g], hi = g + 1 < ngcf ? h[g + 1] : h[ngcf];
+        if (hi <= lo) continue;
+        GridArgs a = whole;
+        a.uv = (const char*)guv + (size_t)lo * ce;
+        a.vals = (const char*)gvals + (size_t)lo * ce;
+        a.gcf = (const char*)job->gcf + (size_t)g * gcf_elems * ce;
+        a.npts = hi - lo;
+        args.push_back(a);
+    }
+    if (args.empty()) {  // no visibilities: a fresh grid is only cleared
+        if (o.accumulate) return MIGRID_OK;
+        const size_t gb = (size_t)whole.N * whole.N * 2 * (whole.out_type == MIGRID_F64 ? 8 : 4);
+        MG_TRY(launch_zero(whole.grid, gb, s));
+        if (whole.grid_lo) MG_TRY(launch_zero(whole.grid_lo, gb, s));
+        return MIGRID_OK;
+    }
+    TiledPlan p;
+    MG_TRY(tiled_plan(args.data(), (int)args.size(), o, p));
+    MG_TRY(ctx->bins.ensure(p.bin_bytes));
+    MG_TRY(ctx->records.ensure(p.arena_bytes));
+    char* arena = (char*)ctx->records.p;
+    std::vector<JobDesc> d;
+    tiled_descs(p, arena, d);
+    MG_TRY(tiled_write_descs(p, arena, d, s));
+    return run_plan(p, args, o, arena, ctx->bins.p, &ctx->prof, s);
+}

@@ -42,6 +42,19 @@ size_t bin_scratch_bytes(int64_t E, int64_t K);
 int bin_sort(const BinJob* jobs, int njobs, int64_t max_npts, int64_t E, int64_t K, int in_type,
              void* scratch, int32_t* perm, int32_t* offsets, hipStream_t s);
 
+// The same sort for keys the caller has written (E keys, each < K) into
+// sort_key_buffer(scratch); scratch as bin_scratch_bytes(E, K).
+uint32_t* sort_key_buffer(void* scratch);
+int sort_keys(int64_t E, int64_t K, void* scratch, int32_t* perm, int32_t* offsets, hipStream_t s);
+
+// Groups npts visibilities by a GCF index in [0, ngcf) (k_inputs.hip):
+// uv_out/vals_out receive them in group order, stable within a group, and
+// offsets[0..ngcf] the group starts (offsets[ngcf] starts the indexes out of
+// range).  scratch: bin_scratch_bytes(npts, ngcf + 1); perm: npts ints.
+int group_by_index(const int32_t* idx, int64_t npts, int ngcf, int in_type, const void* uv,
+                   const void* vals, void* uv_out, void* vals_out, void* scratch, int32_t* perm,
+                   int32_t* offsets, hipStream_t s);
+
 int64_t scan_scratch_elems(int64_t n);
 int exclusive_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* partial, bool write_total,
                    hipStream_t s);

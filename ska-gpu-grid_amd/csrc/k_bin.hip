@@ -391,6 +391,24 @@ size_t bin_scratch_bytes(int64_t E, int64_t K)
 int bin_sort(const BinJob* jobs, int njobs, int64_t max_npts, int64_t E, int64_t K, int in_type,
              void* scratch, int32_t* perm, int32_t* offsets, hipStream_t s)
 {
+    if (E > 0) {
+        uint32_t* k0 = sort_key_buffer(scratch);
+        const int64_t per_job = (max_npts + 255) / 256;
+        const int64_t cap = njobs < 16384 ? 16384 / njobs : 1;
+        const int bx = (int)(per_job < 1 ? 1 : per_job < cap ? per_job : cap);
+        if (in_type == MIGRID_F64)
+            hipLaunchKernelGGL(k_keys<double>, dim3(bx, njobs), dim3(256), 0, s, jobs, k0);
+        else
+            hipLaunchKernelGGL(k_keys<float>, dim3(bx, njobs), dim3(256), 0, s, jobs, k0);
+        MG_HIP(hipGetLastError());
+    }
+    return sort_keys(E, K, scratch, perm, offsets, s);
+}
+
+uint32_t* sort_key_buffer(void* scratch) { return (uint32_t*)scratch; }
+
+int sort_keys(int64_t E, int64_t K, void* scratch, int32_t* perm, int32_t* offsets, hipStream_t s)
+{
     const int64_t nblk = (E + kRsTile - 1) / kRsTile;
     char* p = (char*)scratch;
     auto take = [&](size_t bytes) {
@@ -411,14 +429,6 @@ int bin_sort(const BinJob* jobs, int njobs, int64_t max_npts, int64_t E, int64_t
     // offsets start as the marks array: every key "first seen at E"
     MG_TRY(fill_u32(offsets, (uint32_t)E, K, s));
     if (E > 0) {
-        const int64_t per_job = (max_npts + 255) / 256;
-        const int64_t cap = njobs < 16384 ? 16384 / njobs : 1;
-        const int bx = (int)(per_job < 1 ? 1 : per_job < cap ? per_job : cap);
-        if (in_type == MIGRID_F64)
-            hipLaunchKernelGGL(k_keys<double>, dim3(bx, njobs), dim3(256), 0, s, jobs, k0);
-        else
-            hipLaunchKernelGGL(k_keys<float>, dim3(bx, njobs), dim3(256), 0, s, jobs, k0);
-        MG_HIP(hipGetLastError());
         int bits = 1;
         while (((int64_t)1 << bits) < K) bits++;  // keys are < K
         const int passes = (bits + 7) / 8;

@@ -34,6 +34,7 @@
 // q_y - S + 1 + r) with coefficient G[k][r][c].
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -236,6 +237,7 @@ __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs,
     const int32_t* __restrict__ tcount = jobs[job].tcount;
     uint8_t* __restrict__ split = (uint8_t*)jobs[job].split;
     const bool fresh = jobs[job].fresh != 0;
+    const bool shared = jobs[job].shared != 0;
     const int lane = threadIdx.x & 63;
     const int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t ntiles = (int64_t)g.ntx * g.ntx;
@@ -288,7 +290,7 @@ __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs,
         it.pad = 0;
         items[slot] = it;
     }
-    if (lane == 0) split[tile] = n_items > 1;
+    if (lane == 0) split[tile] = n_items > 1 || shared;
     if (fresh && n_items > 1) {
         zero_tile((TO*)jobs[job].grid, g.N, tx * g.T, ty * g.T, g.T, planar != 0);
         if (jobs[job].grid_lo)
@@ -911,7 +913,14 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.target = o.strategy == MIGRID_TILED ? (o.vis_per_wave > 0 ? o.vis_per_wave : 4096) : 0;
     p.stage_min = o.stage_min;
     p.wg_per_cu = o.wg_per_cu;
-    p.fresh = !o.accumulate;
+    // jobs that write one grid flush atomically, into a grid cleared beforehand
+    {
+        std::vector<const void*> grids;
+        for (const TiledJob& t : p.jobs) grids.push_back(t.a.grid);
+        std::sort(grids.begin(), grids.end());
+        p.shared = std::adjacent_find(grids.begin(), grids.end()) != grids.end();
+    }
+    p.fresh = !o.accumulate && !p.shared;
     const size_t gsz = p.key.stage == kStageBf16 ? 4 : p.key.stage == kStageF32 ? 8 : 16;
     p.lds = (size_t)stage_rows(p.smax) * stage_pitch(p.smax) * gsz;
     return MIGRID_OK;
@@ -933,7 +942,7 @@ void tiled_descs(const TiledPlan& p, char* arena, std::vector<JobDesc>& d)
         d[j].sigma = t.a.sigma;
         d[j].beta = t.a.beta;
         d[j].fresh = p.fresh ? 1 : 0;
-        d[j].pad = 0;
+        d[j].shared = p.shared ? 1 : 0;
     }
 }
 

@@ -60,9 +60,10 @@ struct JobDesc {
     void* grid;
     void* grid_lo;
     double sigma, beta;
-    int32_t fresh;  // grid not cleared beforehand: unsplit tiles are stored,
-                    // empty and split tiles were zeroed by the work-item pass
-    int32_t pad;
+    int32_t fresh;   // grid not cleared beforehand: unsplit tiles are stored,
+                     // empty and split tiles were zeroed by the work-item pass
+    int32_t shared;  // another job of the launch writes the same grid: every
+                     // tile is flushed with atomics
 };
 
 // ---- host-side plan ---------------------------------------------------------
@@ -135,6 +136,7 @@ struct TiledPlan {
     int64_t max_npts = 0;  // the largest job
     int vt = 0;            // record value transform (shared)
     bool fresh = false;    // fresh grids (cleared by the plan's own passes)
+    bool shared = false;   // jobs share grids (atomic flushes; the caller clears)
     size_t off_perm = 0, off_offs = 0, off_rec = 0;
     size_t off_bjobs = 0, off_pjobs = 0, off_cjobs = 0;
     int nconv = 0;             // jobs whose GCF is converted

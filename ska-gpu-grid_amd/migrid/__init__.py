@@ -103,6 +103,7 @@ EXPORTS = [
     "migrid_server_cfg_default", "migrid_server_create", "migrid_server_submit", "migrid_server_poll",
     "migrid_server_wait", "migrid_server_read_grid", "migrid_server_drop_job",
     "migrid_server_get_stats", "migrid_server_destroy",
+    "migrid_uvw_to_pixels", "migrid_ctx_grid_indexed", "migrid_read_uvw_text",
 ]
 
 _lib = None
@@ -160,6 +161,10 @@ def lib():
     L.migrid_server_drop_job.argtypes = [vp, i64]
     L.migrid_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
     L.migrid_server_destroy.argtypes = [vp]
+    L.migrid_uvw_to_pixels.argtypes = [vp, i64, vp, i32, i64, ctypes.c_double, i32, ctypes.c_double,
+                                       vp, vp, vp]
+    L.migrid_ctx_grid_indexed.argtypes = [vp, ctypes.POINTER(Job), i32, vp, ctypes.POINTER(Opts), vp]
+    L.migrid_read_uvw_text.argtypes = [ctypes.c_char_p, i64, i64, vp, vp, ctypes.POINTER(i64)]
     _lib = L
     return L
 
@@ -327,6 +332,21 @@ class Context:
             check(n)
         return [buf[i] for i in range(min(n, cap))]
 
+    def grid_indexed(self, grid, uv, vals, gcfs, gcf_index, N, S, stream=None, **kw):
+        """Gridding with a GCF table per visibility: gcfs (ngcf*64*S*S, 2) or
+        (ngcf, 64*S*S, 2), gcf_index (n,) int32 device tensor."""
+        import torch
+        ngcf = gcfs.numel() // (64 * S * S * 2)
+        if ngcf * 64 * S * S * 2 != gcfs.numel():
+            raise MigridError(-1, "gcfs must hold a whole number of 64*S*S tables")
+        if gcf_index.dtype != torch.int32 or not gcf_index.is_cuda or not gcf_index.is_contiguous():
+            raise MigridError(-1, "gcf_index must be a contiguous int32 device tensor")
+        j = make_job(grid, uv, vals, gcfs, N, S)
+        o = make_opts(**{"strategy": "tiled", **kw})
+        s = ctypes.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
+        check(lib().migrid_ctx_grid_indexed(self.h, ctypes.byref(j), ngcf, gcf_index.data_ptr(),
+                                            ctypes.byref(o), s))
+
     def bin_sort(self, uv, N, S, bin_size, stream=None):
         """Device bin sort; returns (perm, offsets) torch int32 tensors."""
         import torch
@@ -342,6 +362,36 @@ class Context:
                                         offs.data_ptr(), ctypes.byref(cnt),
                                         ctypes.c_void_p(stream.cuda_stream)))
         return perm[: cnt.value], offs
+
+
+def read_uvw_text(path, N, max_rows=None):
+    """The reference's UVW text format -> (uv (n,2) pixels, w (n,)) numpy."""
+    if max_rows is None:
+        with open(path) as f:
+            max_rows = sum(1 for _ in f)
+    uv = np.empty((max(max_rows, 1), 2), np.float64)
+    w = np.empty(max(max_rows, 1), np.float64)
+    n = ctypes.c_int64()
+    check(lib().migrid_read_uvw_text(os.fsencode(path), N, max_rows, uv.ctypes.data, w.ctypes.data,
+                                     ctypes.byref(n)))
+    return uv[: n.value], w[: n.value]
+
+
+def uvw_to_pixels(uvw, freq, N, cell, nplanes=0, w_max=1.0, stream=None):
+    """Device tensors uvw (nrow,3) metres and freq (nchan,) Hz (float64) ->
+    (uv (nrow*nchan, 2) pixels, w-plane index (nrow*nchan,) int32 or None)."""
+    import torch
+    for t in (uvw, freq):
+        if not t.is_cuda or not t.is_contiguous() or t.dtype != torch.float64:
+            raise MigridError(-1, "uvw and freq must be contiguous float64 device tensors")
+    nrow, nchan = uvw.shape[0], freq.shape[0]
+    uv = torch.empty((nrow * nchan, 2), dtype=torch.float64, device=uvw.device)
+    g = torch.empty(nrow * nchan, dtype=torch.int32, device=uvw.device) if nplanes > 0 else None
+    s = ctypes.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
+    check(lib().migrid_uvw_to_pixels(uvw.data_ptr(), nrow, freq.data_ptr(), nchan, N, float(cell),
+                                     nplanes, float(w_max), uv.data_ptr(),
+                                     g.data_ptr() if g is not None else None, s))
+    return uv, g
 
 
 def make_job(grid, uv, vals, gcf, N, S, grid_lo=None):

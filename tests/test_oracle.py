@@ -288,3 +288,30 @@ def test_float_pairs_close_to_double():
     G = gcf.astype(np.float32).reshape(64, S, S, 2)
     assert zc.dtype == np.float64 and cc.dtype == np.float32
     assert G.shape[0] == 64
+
+
+def test_uvw_to_pixels_closed_forms():
+    c = 299792458.0
+    N, cell = 1024, 2.0 ** -20
+    # f = c makes u_lambda = u; one uv cell is 1 / (N cell) = 1024 lambda -> one pixel
+    uvw = np.array([[0.0, 0.0, 0.0], [1024.0, -2048.0, 5.0], [-512.0, 3072.0, -7.0]])
+    uv, _ = O.uvw_to_pixels(uvw, np.array([c]), N, cell)
+    np.testing.assert_array_equal(uv, [[512, 512], [513, 510], [511.5, 515]])
+    # visibility n = row * nchan + chan; the offset from the centre scales with f
+    f = np.array([1.0e8, 2.0e8, 3.0e8])
+    uv, _ = O.uvw_to_pixels(uvw, f, N, cell)
+    d = (uv - N / 2).reshape(3, 3, 2)
+    # (the offset is read back as uv - N/2, so compare to the pixel's rounding)
+    np.testing.assert_allclose(d[1, 1], 2 * d[1, 0], rtol=1e-11)
+    np.testing.assert_allclose(d[2, 2], 3 * d[2, 0], rtol=1e-11)
+    assert d[1, 0, 0] > 0 and d[1, 0, 1] < 0  # u -> x, v -> y, signs kept
+    # 1024 m at 100 MHz is 1024e8/c lambda = 1e8/c uv cells of 1024 lambda
+    np.testing.assert_allclose(d[1, 0, 0], 1e8 / c, rtol=1e-11)
+
+
+def test_uvw_to_pixels_w_planes():
+    c = 299792458.0
+    w = np.array([0.0, 24.999, 25.0, 49.9, 50.0, -74.0, 99.9, 100.0, 1e6])
+    uvw = np.stack([np.zeros_like(w), np.zeros_like(w), w], 1)
+    _, g = O.uvw_to_pixels(uvw, np.array([c]), 256, 1e-3, nplanes=4, w_max=100.0)
+    np.testing.assert_array_equal(g, [0, 0, 1, 1, 2, 2, 3, 3, 3])
