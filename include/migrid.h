@@ -246,6 +246,18 @@ int migrid_uvw_to_pixels(const double* uvw, int64_t nrow, const double* freq, in
 int migrid_ctx_grid_indexed(migrid_ctx* ctx, const migrid_job* job, int32_t ngcf,
                             const int32_t* gcf_index, const migrid_opts* opts, void* stream);
 
+/* Degridding, the adjoint of gridding (no counterpart in the reference):
+ *   vals_out[n] = sum over the in-grid taps of visibility n of
+ *                 conj(G[k][r][c]) * grid[main_y - S/2 + 1 + r][main_x - S/2 + 1 + c]
+ * with the gridder's index math, so <grid(v), g> = <v, degrid(g)>.  job->grid
+ * is read (N x N complex, interleaved unless opts->layout is PLANAR),
+ * job->vals is ignored and vals_out (npts complex of in_type) is written;
+ * visibilities whose window misses the grid get 0.  GCF mode TABLE; arith
+ * NATIVE (double products for double visibilities) or F32.  S <= 64.
+ * Asynchronous on stream. */
+int migrid_ctx_degrid(migrid_ctx* ctx, const migrid_job* job, void* vals_out,
+                      const migrid_opts* opts, void* stream);
+
 /* The reference's UVW text input (grid.cu:338-366): lines
  * "%d,%d,%d: %f, %f, %f" (three integers, u, v, w), u and v scaled to the
  * grid as the reference does, x = (double)(float)(u * (float)img_dim) / 2048.

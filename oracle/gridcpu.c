@@ -519,3 +519,38 @@ void oracle_uvw_to_pixels(const double* uvw, int64_t nrow, const double* freq, i
             }
         }
 }
+
+/* Degridding, the adjoint of gridCPU (no counterpart in the reference):
+ * vals[n] = sum over the in-grid taps of visibility n of
+ *           conj(G[k][r][c]) * grid[main_y - S/2 + 1 + r][main_x - S/2 + 1 + c],
+ * with the index math of gridCPU (exact floor).  grid is planar (2, N, N).
+ * Pinned by <grid(v), g> = <v, degrid(g)> against oracle_grid_zz and by
+ * single-pixel grids. */
+void oracle_degrid_zz(double* vals, const double* uv, int64_t npts, const double* grid, int64_t N,
+                      const double* gcf, int64_t S)
+{
+    const double* re = grid;
+    const double* im = grid + N * N;
+    const int64_t h = S / 2;
+    for (int64_t n = 0; n < npts; n++) {
+        int mx, sx, my, sy;
+        index_exact(uv[2 * n], &mx, &sx);
+        index_exact(uv[2 * n + 1], &my, &sy);
+        const double* g = gcf + 2 * (int64_t)(8 * sy + sx) * S * S;
+        double ar = 0.0, ai = 0.0;
+        for (int64_t r = 0; r < S; r++) {
+            const int64_t y = my - h + 1 + r;
+            if (y < 0 || y >= N) continue;
+            for (int64_t c = 0; c < S; c++) {
+                const int64_t x = mx - h + 1 + c;
+                if (x < 0 || x >= N) continue;
+                const double gr = g[2 * (r * S + c)], gi = g[2 * (r * S + c) + 1];
+                const double xr = re[x + N * y], xi = im[x + N * y];
+                ar += gr * xr + gi * xi;
+                ai += gr * xi - gi * xr;
+            }
+        }
+        vals[2 * n] = ar;
+        vals[2 * n + 1] = ai;
+    }
+}

@@ -63,6 +63,7 @@ def lib():
         L.oracle_window_pixel.restype = _i64
         L.oracle_compute_gcf_phase.argtypes = [_dp, _i64, ctypes.c_double, ctypes.c_double, ctypes.c_int, _fp, _fp]
         L.oracle_grid_const_tap.argtypes = [_dp, _dp, _dp, _i64, _i64, _i64, _fp, _fp]
+        L.oracle_degrid_zz.argtypes = [_dp, _dp, _i64, _dp, _i64, _dp, _i64]
         L.oracle_uvw_to_pixels.argtypes = [_dp, _i64, _dp, _i64, _i64, ctypes.c_double, _i64,
                                            ctypes.c_double, _dp, _ip]
         _lib = L
@@ -246,6 +247,17 @@ def grid_const_tap(uv, vals, N, S, r2, i2):
     lib().oracle_grid_const_tap(_p(out, _dp), _p(uv, _dp), _p(vals, _dp), uv.shape[0], N, S,
                                 _p(r2, _fp), _p(i2, _fp))
     return out
+
+
+def degrid_zz(uv, grid, N, gcf, S):
+    """Adjoint of grid_zz: grid planar (2,N,N) -> values (n,2)."""
+    uv = _c(uv, np.float64).reshape(-1, 2)
+    grid = _c(grid, np.float64).reshape(2, N, N)
+    gcf = _c(gcf, np.float64).reshape(-1, 2)
+    vals = np.empty((uv.shape[0], 2), np.float64)
+    lib().oracle_degrid_zz(_p(vals, _dp), _p(uv, _dp), uv.shape[0], _p(grid, _dp), N,
+                           _p(gcf, _dp), S)
+    return vals
 
 
 def uvw_to_pixels(uvw, freq, N, cell, nplanes=0, w_max=1.0):

@@ -104,6 +104,7 @@ EXPORTS = [
     "migrid_server_wait", "migrid_server_read_grid", "migrid_server_drop_job",
     "migrid_server_get_stats", "migrid_server_destroy",
     "migrid_uvw_to_pixels", "migrid_ctx_grid_indexed", "migrid_read_uvw_text",
+    "migrid_ctx_degrid",
 ]
 
 _lib = None
@@ -165,6 +166,7 @@ def lib():
                                        vp, vp, vp]
     L.migrid_ctx_grid_indexed.argtypes = [vp, ctypes.POINTER(Job), i32, vp, ctypes.POINTER(Opts), vp]
     L.migrid_read_uvw_text.argtypes = [ctypes.c_char_p, i64, i64, vp, vp, ctypes.POINTER(i64)]
+    L.migrid_ctx_degrid.argtypes = [vp, ctypes.POINTER(Job), vp, ctypes.POINTER(Opts), vp]
     _lib = L
     return L
 
@@ -331,6 +333,18 @@ class Context:
         if n < 0:
             check(n)
         return [buf[i] for i in range(min(n, cap))]
+
+    def degrid(self, grid, uv, gcf, N, S, out=None, stream=None, **kw):
+        """Degridding (the adjoint of grid): visibility values (n, 2) in uv's
+        dtype from the (N,N,2) or planar (2,N,N) grid.  Asynchronous."""
+        import torch
+        if out is None:
+            out = torch.empty((uv.shape[0], 2), dtype=uv.dtype, device=uv.device)
+        j = make_job(grid, uv, out, gcf, N, S)
+        o = make_opts(**kw)
+        s = ctypes.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
+        check(lib().migrid_ctx_degrid(self.h, ctypes.byref(j), out.data_ptr(), ctypes.byref(o), s))
+        return out
 
     def grid_indexed(self, grid, uv, vals, gcfs, gcf_index, N, S, stream=None, **kw):
         """Gridding with a GCF table per visibility: gcfs (ngcf*64*S*S, 2) or

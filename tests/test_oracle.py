@@ -315,3 +315,36 @@ def test_uvw_to_pixels_w_planes():
     uvw = np.stack([np.zeros_like(w), np.zeros_like(w), w], 1)
     _, g = O.uvw_to_pixels(uvw, np.array([c]), 256, 1e-3, nplanes=4, w_max=100.0)
     np.testing.assert_array_equal(g, [0, 0, 1, 1, 2, 2, 3, 3, 3])
+
+
+@pytest.mark.parametrize("S,N", [(8, 40), (16, 64), (32, 96)])
+def test_degrid_is_the_adjoint_of_gridcpu(S, N):
+    """<grid(v), g> = <v, degrid(g)> for random v, g (complex inner products):
+    a sign, conjugation, index or transpose error in either side breaks it."""
+    rng = np.random.default_rng(S)
+    uv, v = O.gen_uniform(300, -S / 2, N + S / 2, seed=S + 1)
+    gcf = rng.normal(size=(64 * S * S, 2))
+    g = rng.normal(size=(2, N, N))
+    Av = O.grid_zz(uv, v, N, gcf, S)
+    lhs = np.vdot(Av[0] + 1j * Av[1], g[0] + 1j * g[1])
+    Ag = O.degrid_zz(uv, g, N, gcf, S)
+    rhs = np.vdot(v[:, 0] + 1j * v[:, 1], Ag[:, 0] + 1j * Ag[:, 1])
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
+
+
+def test_degrid_single_pixel_grid():
+    """A grid with one unit pixel p returns, for each visibility, conj of the
+    tap of its sub-kernel that lands on p (zero when none does)."""
+    S, N = 8, 32
+    gcf = O.init_gcf(S)
+    G = gcf.reshape(64, S, S, 2)
+    g = np.zeros((2, N, N))
+    g[0, 13, 17] = 1.0
+    uv = np.array([[17.3, 13.8], [20.0, 10.5], [5.0, 5.0], [16.99, 13.0]])
+    got = O.degrid_zz(uv, g, N, gcf, S)
+    for n, (u, v) in enumerate(uv):
+        mx, my = int(np.floor(u)), int(np.floor(v))
+        k = 8 * int(np.floor(8 * (v - my))) + int(np.floor(8 * (u - mx)))
+        c, r = 17 - (mx - S // 2 + 1), 13 - (my - S // 2 + 1)
+        want = G[k, r, c, 0] - 1j * G[k, r, c, 1] if (0 <= c < S and 0 <= r < S) else 0
+        assert got[n, 0] + 1j * got[n, 1] == want
