@@ -329,6 +329,61 @@ def main_serve(args, torch, migrid):
     print(json.dumps(out), flush=True)
 
 
+def main_degrid(args, torch, migrid):
+    """Degridding (the adjoint of gridding) of the gridding workload: the
+    same uv onto a complex N(0,1) grid drawn on the device, FP64 and FP32
+    products.  A step is one full pass: zero fill, sort by (tile, sub-kernel)
+    and the degridding kernel."""
+    N, S, npts = args.N, args.S, args.npts
+    uv, _ = gen_clustered(npts, N, S) if args.dist == "clustered" else gen_uniform(npts, N, S)
+    gcf = config_gcf(S)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(args.seed)
+    d_uv = torch.from_numpy(uv).cuda()
+    d_gcf = torch.from_numpy(gcf).cuda()
+    grid = torch.randn((N, N, 2), dtype=torch.float64, device="cuda", generator=gen)
+    out = torch.empty((npts, 2), dtype=torch.float64, device="cuda")
+    ctx = migrid.Context()
+    nchk = min(npts, 4000)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # CPU restatement, the checker of the first visibilities
+    hgrid = grid.permute(2, 0, 1).contiguous().cpu().numpy()
+    ref = O.degrid_zz(uv[:nchk], hgrid, N, gcf, S)
+    results = {}
+    for name, arith in (("fp64", migrid.ARITH_NATIVE), ("fp32", migrid.ARITH_F32)):
+        def step(arith=arith):
+            ctx.degrid(grid, d_uv, d_gcf, N, S, out=out, arith=arith, tile=args.tile,
+                       vis_per_wave=args.vpw, wg_per_cu=args.wg_per_cu)
+        print(f"# degrid {name}: timing {args.steps} steps", file=sys.stderr, flush=True)
+        wall, per, kern = time_variant(torch, ctx, step, args.warmup, args.steps)
+        got = out[:nchk].cpu().numpy()
+        err = float(np.abs(got - ref).max() / np.abs(ref).max())
+        ms = wall * 1e3 / args.steps
+        results[name] = {"ms_per_step": ms, "mvis_per_s": npts / (ms * 1e-3) / 1e6,
+                         "p50_ms": pct(per, 50), "p99_ms": pct(per, 99),
+                         "kernel_ms_avg": float(np.mean(kern)) if kern else None,
+                         "rel_max_vs_oracle_first_%d" % nchk: err}
+        print(f"# degrid {name}: {results[name]}", file=sys.stderr, flush=True)
+    head = "fp32"
+    R = results[head]
+    flops = 8.0 * S * S * npts
+    achieved = flops / (R["kernel_ms_avg"] * 1e-3) / 1e12
+    out_line = {
+        "metric": f"Mvis/s degridding at {N}² grid, support {S}",
+        "value": R["mvis_per_s"], "unit": "Mvis/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": R["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"degridding {N}x{N} complex128 grid, support {S}, oversampling 8, "
+                               f"{npts} radially clustered visibilities (Exp radius mean 800 px)",
+                   "variant": head},
+        "variants": results,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TF,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TF, "traffic": None,
+                     "kernel": "degridding kernel", "flops_per_launch": flops},
+    }
+    print(json.dumps(out_line), flush=True)
+
+
 def main_copy(args, torch):
     """Plain device-to-device copy of 1 GiB: the calibration kernel for the
     HBM byte counters (it reads and writes exactly 2^30 bytes per launch)."""
@@ -374,10 +429,11 @@ def main():
     ap.add_argument("--stage-min", type=int, default=0, help="tiled: staging threshold")
     ap.add_argument("--wg-per-cu", type=int, default=0, help="tiled: workgroups per CU")
     ap.add_argument("--dist", default="clustered", choices=["clustered", "uniform"])
-    ap.add_argument("--mode", default="grid", choices=["grid", "batch", "serve", "copy"],
+    ap.add_argument("--mode", default="grid", choices=["grid", "batch", "serve", "copy", "degrid"],
                     help="grid: one 8192^2 job (default); batch: many jobs in one persistent "
                          "launch; serve: Poisson request stream through the serving loop; "
-                         "copy: 1 GiB device copy (byte-counter calibration)")
+                         "copy: 1 GiB device copy (byte-counter calibration); degrid: the "
+                         "adjoint (grid -> visibilities) of the grid workload")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--jobs", type=int, default=64, help="batch/serve: job ids")
     ap.add_argument("--large-jobs", type=int, default=4, help="serve: 8192^2 job ids")
@@ -402,6 +458,8 @@ def main():
         return main_serve(args, torch, migrid)
     if args.mode == "copy":
         return main_copy(args, torch)
+    if args.mode == "degrid":
+        return main_degrid(args, torch, migrid)
     N, S, npts = args.N, args.S, args.npts
     t0 = time.perf_counter()
     uv, vals = gen_clustered(npts, N, S) if args.dist == "clustered" else gen_uniform(npts, N, S)

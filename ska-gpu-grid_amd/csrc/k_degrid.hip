@@ -7,12 +7,16 @@
 // with the gridder's index math, so <grid(v), g> = <v, degrid(g)>.
 //
 // MI355X layout: a visibility belongs to the tile that holds its window's
-// corner, q = floor(uv) + S/2 binned by the tile edge T (the gridder's q).
-// One workgroup per non-empty tile stages the tile and its (S - 1)-pixel halo
-// of the grid in LDS once (zeros off the grid), then each wave takes whole
-// visibilities: its lanes walk the sub-kernel's columns, reading a GCF row
-// (coalesced, cache-resident: the tile's visibilities are sorted by sub-kernel)
-// and the matching staged grid row, and a 64-lane reduction gives the value.
+// corner, q = floor(uv) + S/2 binned by the tile edge T (the gridder's q),
+// and the visibilities are sorted by (tile, sub-kernel).  A tile's run is cut
+// into work items of at most `target` visibilities, so a dense tile is shared
+// by many workgroups.  Persistent workgroups take items in turn: stage the
+// item's tile and its (S - 1)-pixel halo of the grid in LDS (zeros off the
+// grid), then each wave takes groups of V consecutive visibilities: its lanes
+// walk the sub-kernel's columns, reading a GCF row (coalesced) and the
+// matching staged grid row of each visibility, and a 64-lane reduction gives
+// each value.  A group whose visibilities share the sub-kernel (the common
+// case in a dense tile, being sorted by it) reads each GCF row once for all V.
 // Every visibility is computed by exactly one wave, so no atomics.
 #include <hip/hip_runtime.h>
 
@@ -25,13 +29,18 @@
 namespace migrid {
 namespace {
 
+struct DegridItem {
+    int32_t tile, lo, hi, pad;  // sorted positions [lo, hi) of one tile
+};
+
 struct DegridArgs {
     const void* uv;
     const void* gtab;  // 64 * S^2 complex in the product type
     const void* grid;
     void* out;
-    const int32_t* perm;
-    const int32_t* offs;  // CSR over (tile, k) keys
+    const int4* rec;  // per sorted position: {n, tile offset of the window, k, 0}
+    const DegridItem* items;
+    const int32_t* nitems;  // device count of items
     int N, S, T, ntx;
     int planar;
 };
@@ -77,68 +86,205 @@ __global__ __launch_bounds__(256) void k_degrid_keys(const TI* __restrict__ uv, 
     }
 }
 
-// G: coefficient pair type, GS: staged grid pair type, TA: accumulation.
-template <typename G, typename GS, typename TA, typename TI, typename TO>
-__global__ __launch_bounds__(1024) void k_degrid(DegridArgs a)
+// Items per tile: ceil(count / target).
+__global__ __launch_bounds__(256) void k_degrid_item_counts(const int32_t* __restrict__ offs,
+                                                            int64_t ntiles, int target,
+                                                            int32_t* __restrict__ ni)
 {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    GS* tile = (GS*)smem_raw;
-    const int t = blockIdx.x;
-    const int32_t r0 = a.offs[(int64_t)t * kSubKernels];
-    const int32_t r1 = a.offs[(int64_t)(t + 1) * kSubKernels];
-    if (r0 >= r1) return;
-    const int N = a.N, S = a.S, T = a.T;
-    const int tx = t % a.ntx, ty = t / a.ntx;
-    const int X0 = tx * T - (S - 1), Y0 = ty * T - (S - 1);
-    const int W = T + S - 1;
-    const TO* __restrict__ grid = (const TO*)a.grid;
-    const int64_t NN = (int64_t)N * N;
-    for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
-        const int yy = e / W, xx = e - yy * W;
-        const int y = Y0 + yy, x = X0 + xx;
-        GS g;
-        g.re = 0;
-        g.im = 0;
-        if (x >= 0 && y >= 0 && x < N && y < N) {
-            const int64_t p = (int64_t)y * N + x;
-            g.re = (decltype(g.re))(a.planar ? grid[p] : grid[2 * p]);
-            g.im = (decltype(g.im))(a.planar ? grid[NN + p] : grid[2 * p + 1]);
-        }
-        tile[e] = g;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntiles;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t cnt = offs[(t + 1) * kSubKernels] - offs[t * kSubKernels];
+        ni[t] = (cnt + target - 1) / target;
     }
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int R = 64 / S;          // rows a pass covers (S <= 64)
-    const int c = lane % S, rr = lane / S;
-    const bool active = rr < R;
-    const TI* __restrict__ uv = (const TI*)a.uv;
-    const G* __restrict__ gtab = (const G*)a.gtab;
-    TI* __restrict__ out = (TI*)a.out;
-    for (int j = r0 + wave; j < r1; j += nw) {
-        const int n = __builtin_amdgcn_readfirstlane(a.perm[j]);
+}
+
+// A tile's run split into ni[t] nearly equal items starting at istart[t].
+__global__ __launch_bounds__(256) void k_degrid_items(const int32_t* __restrict__ offs,
+                                                      const int32_t* __restrict__ ni,
+                                                      const int32_t* __restrict__ istart,
+                                                      int64_t ntiles, DegridItem* __restrict__ items)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntiles;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t m = ni[t];
+        const int32_t a = offs[t * kSubKernels];
+        const int64_t cnt = offs[(t + 1) * kSubKernels] - a;
+        DegridItem* dst = items + istart[t];
+        for (int32_t i = 0; i < m; ++i) {
+            DegridItem it;
+            it.tile = (int32_t)t;
+            it.lo = a + (int32_t)(cnt * i / m);
+            it.hi = a + (int32_t)(cnt * (i + 1) / m);
+            it.pad = 0;
+            dst[i] = it;
+        }
+    }
+}
+
+// Records in sorted order for the visibilities that fall in a tile (the
+// first offs_end[0] sorted positions): the index n, the offset of the
+// window's corner in the staged tile (W pixels a row) and the sub-kernel k.
+template <typename TI>
+__global__ __launch_bounds__(256) void k_degrid_pack(const int32_t* __restrict__ perm,
+                                                     const TI* __restrict__ uv,
+                                                     const int32_t* __restrict__ offs_end, int S,
+                                                     int T, int4* __restrict__ rec)
+{
+    const int64_t e = *offs_end;
+    const int W = T + S - 1;
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < e;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const int n = perm[j];
         const TI u = uv[2 * (int64_t)n], v = uv[2 * (int64_t)n + 1];
         const TI fu = floor_t(u), fv = floor_t(v);
         const int k = kOversample * (int)floor_t((TI)kOversample * (v - fv)) +
                       (int)floor_t((TI)kOversample * (u - fu));
-        const int xr = (int)fu + S / 2 - tx * T;  // q - tile origin, in [0, T)
-        const int yr = (int)fv + S / 2 - ty * T;
-        const G* __restrict__ gk = gtab + (int64_t)k * S * S;
-        TA ar = 0, ai = 0;
-        if (active)
-            for (int r = rr; r < S; r += R) {
-                const G g = gk[r * S + c];
-                const GS x = tile[(yr + r) * W + xr + c];
-                const TA gr = (TA)g.x, gi = (TA)g.y, xr_ = (TA)x.re, xi = (TA)x.im;
-                ar += gr * xr_ + gi * xi;
-                ai += gr * xi - gi * xr_;
+        const int qx = (int)fu + S / 2, qy = (int)fv + S / 2;  // in [0, N + S - 2]
+        const int xr = qx - (qx / T) * T, yr = qy - (qy / T) * T;
+        rec[j] = make_int4(n, yr * W + xr, k, 0);
+    }
+}
+
+// Sums of conj(G[k]) x window over the sub-kernel rows of this lane for VV
+// visibilities sharing gk (this lane's column of sub-kernel k) at staged
+// offsets off[] (this lane's column included).  Rows go in blocks of U; the
+// next block's coefficients are loaded while the current block computes.
+template <int VV, int U, typename G, typename GS, typename TA>
+__device__ __forceinline__ void degrid_rows(const G* __restrict__ gk, const GS* tile,
+                                            const int* off, int rr, int R, int S, int W, TA* ar,
+                                            TA* ai)
+{
+    const int step = U * R;
+    if (S % step == 0) {  // every active lane has S / R rows, in whole blocks
+        G gn[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) gn[u] = gk[(rr + u * R) * S];
+        for (int r0 = rr; r0 < S; r0 += step) {
+            G g[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) g[u] = gn[u];
+            if (r0 + step < S)
+#pragma unroll
+                for (int u = 0; u < U; ++u) gn[u] = gk[(r0 + step + u * R) * S];
+            GS x[U][VV];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int i = 0; i < VV; ++i) x[u][i] = tile[off[i] + (r0 + u * R) * W];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const TA gr = (TA)g[u].x, gi = (TA)g[u].y;
+#pragma unroll
+                for (int i = 0; i < VV; ++i) {
+                    const TA xr = (TA)x[u][i].re, xi = (TA)x[u][i].im;
+                    ar[i] = fma(gr, xr, ar[i]);
+                    ar[i] = fma(gi, xi, ar[i]);
+                    ai[i] = fma(gr, xi, ai[i]);
+                    ai[i] = fma(-gi, xr, ai[i]);
+                }
             }
-        ar = wave_sum(ar);
-        ai = wave_sum(ai);
-        if (lane == 0) {
-            out[2 * (int64_t)n] = (TI)ar;
-            out[2 * (int64_t)n + 1] = (TI)ai;
         }
+    } else {
+        for (int r = rr; r < S; r += R) {
+            const G g = gk[r * S];
+            const TA gr = (TA)g.x, gi = (TA)g.y;
+#pragma unroll
+            for (int i = 0; i < VV; ++i) {
+                const GS x = tile[off[i] + r * W];
+                const TA xr = (TA)x.re, xi = (TA)x.im;
+                ar[i] = fma(gr, xr, ar[i]);
+                ar[i] = fma(gi, xi, ar[i]);
+                ai[i] = fma(gr, xi, ai[i]);
+                ai[i] = fma(-gi, xr, ai[i]);
+            }
+        }
+    }
+}
+
+// G: coefficient pair type, GS: staged grid pair type, TA: accumulation,
+// V: visibilities per wave pass.
+template <int V, typename G, typename GS, typename TA, typename TI, typename TO>
+__global__ __launch_bounds__(1024) void k_degrid(DegridArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    GS* tile = (GS*)smem_raw;
+    const int N = a.N, S = a.S, T = a.T;
+    const int W = T + S - 1;
+    const TO* __restrict__ grid = (const TO*)a.grid;
+    const int64_t NN = (int64_t)N * N;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nw = blockDim.x >> 6;
+    const int R = 64 / S;  // rows a pass covers (S <= 64)
+    const int c = lane % S, rr = lane / S;
+    const bool active = rr < R;
+    const G* __restrict__ gtab = (const G*)a.gtab;
+    TI* __restrict__ out = (TI*)a.out;
+    const int total = *a.nitems;
+    constexpr int U = sizeof(TA) == 8 ? 2 : 4;  // rows per block
+    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+        const DegridItem item = a.items[it];
+        const int tx = item.tile % a.ntx, ty = item.tile / a.ntx;
+        const int X0 = tx * T - (S - 1), Y0 = ty * T - (S - 1);
+        for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
+            const int yy = e / W, xx = e - yy * W;
+            const int y = Y0 + yy, x = X0 + xx;
+            GS g;
+            g.re = 0;
+            g.im = 0;
+            if (x >= 0 && y >= 0 && x < N && y < N) {
+                const int64_t p = (int64_t)y * N + x;
+                g.re = (decltype(g.re))(a.planar ? grid[p] : grid[2 * p]);
+                g.im = (decltype(g.im))(a.planar ? grid[NN + p] : grid[2 * p + 1]);
+            }
+            tile[e] = g;
+        }
+        __syncthreads();
+        // this wave's contiguous share of the item, in groups of V
+        const int len = item.hi - item.lo;
+        const int w0 = item.lo + (int)((int64_t)len * wave / nw);
+        const int w1 = item.lo + (int)((int64_t)len * (wave + 1) / nw);
+        int4 recn = make_int4(0, 0, 0, 0);
+        if (w0 < w1) recn = a.rec[min(w0 + lane % V, w1 - 1)];
+        for (int j0 = w0; j0 < w1; j0 += V) {
+            const int4 rc = recn;  // lane i: visibility j0 + min(i, nv - 1)
+            const int nv = w1 - j0 < V ? w1 - j0 : V;
+            if (j0 + V < w1) recn = a.rec[min(j0 + V + lane % V, w1 - 1)];
+            int n[V], off[V], k[V];
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                n[i] = __builtin_amdgcn_readlane(rc.x, i);
+                off[i] = __builtin_amdgcn_readlane(rc.y, i) + c;
+                k[i] = __builtin_amdgcn_readlane(rc.z, i);
+            }
+            TA ar[V], ai[V];
+#pragma unroll
+            for (int i = 0; i < V; ++i) ar[i] = ai[i] = 0;
+            if (active) {
+                if (k[0] == k[V - 1]) {  // sorted by k: the whole group shares it
+                    degrid_rows<V, U>(gtab + (int64_t)k[0] * S * S + c, tile, off, rr, R, S, W,
+                                      ar, ai);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < V; ++i)
+                        degrid_rows<1, U>(gtab + (int64_t)k[i] * S * S + c, tile, off + i, rr, R,
+                                          S, W, ar + i, ai + i);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                ar[i] = wave_sum(ar[i]);
+                ai[i] = wave_sum(ai[i]);
+            }
+            if (lane == 0)
+#pragma unroll
+                for (int i = 0; i < V; ++i)
+                    if (i < nv) {
+                        out[2 * (int64_t)n[i]] = (TI)ar[i];
+                        out[2 * (int64_t)n[i] + 1] = (TI)ai[i];
+                    }
+        }
+        __syncthreads();  // the tile is restaged by the next item
     }
 }
 
@@ -163,15 +309,32 @@ int blocks_for(int64_t n)
 
 size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-template <typename G, typename GS, typename TA, typename TI, typename TO>
-int run_degrid(const DegridArgs& a, size_t lds, int64_t ntiles, hipStream_t s)
+template <int V, typename G, typename GS, typename TA, typename TI, typename TO>
+int run_degrid_v(const DegridArgs& a, size_t lds, int wg_per_cu, hipStream_t s)
 {
-    auto kern = k_degrid<G, GS, TA, TI, TO>;
+    auto kern = k_degrid<V, G, GS, TA, TI, TO>;
     MG_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
-    hipLaunchKernelGGL(kern, dim3((unsigned)ntiles), dim3(1024), lds, s, a);
+    int per_cu = 0;
+    MG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 1024, lds));
+    if (per_cu < 1) return fail(MIGRID_ENOTSUP, "degridding kernel does not fit (lds=%zu)", lds);
+    if (wg_per_cu > 0 && wg_per_cu < per_cu) per_cu = wg_per_cu;
+    int dev = 0, ncu = 0;
+    MG_HIP(hipGetDevice(&dev));
+    MG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    hipLaunchKernelGGL(kern, dim3((unsigned)(ncu * per_cu)), dim3(1024), lds, s, a);
     MG_HIP(hipGetLastError());
     return MIGRID_OK;
+}
+
+template <typename G, typename GS, typename TA, typename TI, typename TO>
+int run_degrid(const DegridArgs& a, int V, size_t lds, int wg_per_cu, hipStream_t s)
+{
+    switch (V) {
+    case 1: return run_degrid_v<1, G, GS, TA, TI, TO>(a, lds, wg_per_cu, s);
+    case 2: return run_degrid_v<2, G, GS, TA, TI, TO>(a, lds, wg_per_cu, s);
+    default: return run_degrid_v<4, G, GS, TA, TI, TO>(a, lds, wg_per_cu, s);
+    }
 }
 
 }  // namespace
@@ -210,10 +373,27 @@ extern "C" int migrid_ctx_degrid(migrid_ctx* ctx, const migrid_job* job, void* v
     // the staged grid is kept in the type of the sums
     const bool acc64 = (!f32) || (!in64 && out64);
     const size_t gsz = acc64 ? 16 : 8;
-    int T = 64;
-    while (T > 16 && (size_t)(T + S - 1) * (T + S - 1) * gsz > 160 * 1024) T /= 2;
-    const size_t lds = (size_t)(T + S - 1) * (T + S - 1) * gsz;
-    if (lds > 160 * 1024) return fail(MIGRID_ENOTSUP, "degridding tile does not fit in LDS");
+    auto tile_lds = [&](int t) { return (size_t)(t + S - 1) * (t + S - 1) * gsz; };
+    // the largest tile edge that lets two workgroups share a CU, else the
+    // largest that fits one
+    int T = 0;
+    if (o.tile > 0) {
+        if (o.tile != 16 && o.tile != 32 && o.tile != 64)
+            return fail(MIGRID_EINVAL, "degridding tile %d (16, 32 or 64)", o.tile);
+        T = o.tile;
+    } else {
+        for (int t = 64; t >= 16 && !T; t /= 2)
+            if (tile_lds(t) <= 80 * 1024) T = t;
+        for (int t = 64; t >= 16 && !T; t /= 2)
+            if (tile_lds(t) <= 160 * 1024) T = t;
+    }
+    if (!T || tile_lds(T) > 160 * 1024)
+        return fail(MIGRID_ENOTSUP, "degridding tile does not fit in LDS");
+    const size_t lds = tile_lds(T);
+    const int V = o.vis_per_wave > 0 ? o.vis_per_wave : 4;
+    if (V != 1 && V != 2 && V != 4)
+        return fail(MIGRID_EINVAL, "degridding: vis_per_wave %d (1, 2 or 4)", V);
+    const int target = 2048;  // visibilities per work item
     const int ntx = (N + S - 2) / T + 1;
     const int64_t ntiles = (int64_t)ntx * ntx;
     const int64_t K = ntiles * kSubKernels + 1;
@@ -224,15 +404,34 @@ extern "C" int migrid_ctx_degrid(migrid_ctx* ctx, const migrid_job* job, void* v
     const size_t ce_in = in64 ? 16 : 8;
     MG_TRY(launch_zero(vals_out, (size_t)npts * ce_in, s));  // windows that miss the grid
     if (npts == 0) return MIGRID_OK;
-    // arena: permutation, CSR offsets, the GCF in the product type
+    // arena: permutation, CSR offsets, items per tile and their starts, scan
+    // partials, the items, the records, the GCF in the product type
     const bool convert = f32 && in64;
+    const int64_t max_items = npts / target + ntiles;
+    const size_t b_perm = up256((size_t)npts * 4), b_offs = up256((size_t)K * 4);
+    const size_t b_ni = up256((size_t)ntiles * 4), b_is = up256((size_t)(ntiles + 1) * 4);
+    const size_t b_part = up256((size_t)scan_scratch_elems(ntiles) * 4);
+    const size_t b_items = up256((size_t)max_items * sizeof(DegridItem));
+    const size_t b_rec = up256((size_t)npts * sizeof(int4));
     const size_t gtab_bytes = convert ? (size_t)kSubKernels * S * S * 8 : 0;
-    const size_t need = up256((size_t)npts * 4) + up256((size_t)K * 4) + gtab_bytes;
-    MG_TRY(ctx->records.ensure(need));
+    MG_TRY(ctx->records.ensure(b_perm + b_offs + b_ni + b_is + b_part + b_items + b_rec +
+                               gtab_bytes));
     char* ar = (char*)ctx->records.p;
     int32_t* perm = (int32_t*)ar;
-    int32_t* offs = (int32_t*)(ar + up256((size_t)npts * 4));
-    void* gtab = ar + up256((size_t)npts * 4) + up256((size_t)K * 4);
+    ar += b_perm;
+    int32_t* offs = (int32_t*)ar;
+    ar += b_offs;
+    int32_t* ni = (int32_t*)ar;
+    ar += b_ni;
+    int32_t* istart = (int32_t*)ar;
+    ar += b_is;
+    int32_t* partial = (int32_t*)ar;
+    ar += b_part;
+    DegridItem* items = (DegridItem*)ar;
+    ar += b_items;
+    int4* rec = (int4*)ar;
+    ar += b_rec;
+    void* gtab = ar;
     MG_TRY(ctx->bins.ensure(bin_scratch_bytes(npts, K)));
     if (in64)
         hipLaunchKernelGGL(k_degrid_keys<double>, dim3(blocks_for(npts)), dim3(256), 0, s,
@@ -242,6 +441,20 @@ extern "C" int migrid_ctx_degrid(migrid_ctx* ctx, const migrid_job* job, void* v
                            (const float*)job->uv, npts, N, S, T, ntx, sort_key_buffer(ctx->bins.p));
     MG_HIP(hipGetLastError());
     MG_TRY(sort_keys(npts, K, ctx->bins.p, perm, offs, s));
+    if (in64)
+        hipLaunchKernelGGL(k_degrid_pack<double>, dim3(blocks_for(npts)), dim3(256), 0, s, perm,
+                           (const double*)job->uv, offs + ntiles * kSubKernels, S, T, rec);
+    else
+        hipLaunchKernelGGL(k_degrid_pack<float>, dim3(blocks_for(npts)), dim3(256), 0, s, perm,
+                           (const float*)job->uv, offs + ntiles * kSubKernels, S, T, rec);
+    MG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_degrid_item_counts, dim3(blocks_for(ntiles)), dim3(256), 0, s, offs,
+                       ntiles, target, ni);
+    MG_HIP(hipGetLastError());
+    MG_TRY(exclusive_scan(ni, ntiles, istart, partial, true, s));
+    hipLaunchKernelGGL(k_degrid_items, dim3(blocks_for(ntiles)), dim3(256), 0, s, offs, ni, istart,
+                       ntiles, items);
+    MG_HIP(hipGetLastError());
     if (convert) {
         const int64_t ng = (int64_t)kSubKernels * S * S;
         hipLaunchKernelGGL((k_degrid_convert<double, float2>), dim3(blocks_for(ng)), dim3(256), 0, s,
@@ -253,23 +466,25 @@ extern "C" int migrid_ctx_degrid(migrid_ctx* ctx, const migrid_job* job, void* v
     a.gtab = convert ? gtab : job->gcf;
     a.grid = job->grid;
     a.out = vals_out;
-    a.perm = perm;
-    a.offs = offs;
+    a.rec = rec;
+    a.items = items;
+    a.nitems = istart + ntiles;
     a.N = N;
     a.S = S;
     a.T = T;
     a.ntx = ntx;
     a.planar = o.layout == MIGRID_LAYOUT_PLANAR ? 1 : 0;
+    const int wpc = o.wg_per_cu;
     MG_TRY(ctx->prof.begin(s));
     int rc;
     if (!f32)  // double visibilities, double grid, double products
-        rc = run_degrid<double2, C2<double>, double, double, double>(a, lds, ntiles, s);
+        rc = run_degrid<double2, C2<double>, double, double, double>(a, V, lds, wpc, s);
     else if (in64)  // double visibilities, FP32 tiles
-        rc = out64 ? run_degrid<float2, C2<float>, float, double, double>(a, lds, ntiles, s)
-                   : run_degrid<float2, C2<float>, float, double, float>(a, lds, ntiles, s);
+        rc = out64 ? run_degrid<float2, C2<float>, float, double, double>(a, V, lds, wpc, s)
+                   : run_degrid<float2, C2<float>, float, double, float>(a, V, lds, wpc, s);
     else  // float visibilities: sums in the grid's type
-        rc = out64 ? run_degrid<float2, C2<double>, double, float, double>(a, lds, ntiles, s)
-                   : run_degrid<float2, C2<float>, float, float, float>(a, lds, ntiles, s);
+        rc = out64 ? run_degrid<float2, C2<double>, double, float, double>(a, V, lds, wpc, s)
+                   : run_degrid<float2, C2<float>, float, float, float>(a, V, lds, wpc, s);
     MG_TRY(rc);
     MG_TRY(ctx->prof.end(s));
     return MIGRID_OK;

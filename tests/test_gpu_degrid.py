@@ -45,6 +45,27 @@ def test_degrid_matches_oracle(S, N, npts):
     assert np.abs(got_p - want).max() <= 1e-12 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("tile", [16, 32, 64])
+@pytest.mark.parametrize("vpw", [1, 2, 4])
+def test_degrid_tiles_and_groups(tile, vpw):
+    """Every tile edge and visibilities-per-pass choice; a dense centre puts
+    thousands of visibilities in one tile (split into several work items) and
+    long runs of one sub-kernel (groups sharing the GCF rows)."""
+    N, S = 160, 32
+    rng = np.random.default_rng(tile + vpw)
+    uv, _ = O.gen_uniform(6000, -S, N + S, seed=tile * 10 + vpw)
+    uv[:4000] = N / 2 + rng.normal(scale=6.0, size=(4000, 2))
+    gcf = O.config_gcf(S)
+    g = rng.normal(size=(2, N, N))
+    want = cplx(O.degrid_zz(uv, g, N, gcf, S))
+    ctx = migrid.Context()
+    grid = dev(np.moveaxis(g, 0, -1))
+    for arith, tol in ((migrid.ARITH_NATIVE, 1e-12), (migrid.ARITH_F32, 1e-5)):
+        got = cplx(ctx.degrid(grid, dev(uv), dev(gcf), N, S, arith=arith, tile=tile,
+                              vis_per_wave=vpw).cpu().numpy())
+        assert np.abs(got - want).max() <= tol * np.abs(want).max()
+
+
 def test_degrid_float_types_and_empty():
     rng = np.random.default_rng(5)
     N, S = 128, 16
