@@ -384,6 +384,49 @@ def main_degrid(args, torch, migrid):
     print(json.dumps(out_line), flush=True)
 
 
+def main_image(args, torch, migrid):
+    """Grid -> image: the centred 2-D FFT of an N x N complex grid (the step
+    after gridding).  A step is one full transform: sign/layout pass, hipFFT
+    and the sign/scale/taper pass.  Algorithmic bytes: the grid read once and
+    the image written once."""
+    N = args.N
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(args.seed)
+    ctx = migrid.Context()
+    taper = torch.linspace(0.5, 1.0, N, dtype=torch.float64, device="cuda")
+    results = {}
+    for name, dt, esz in (("fp64", torch.float64, 16), ("fp32", torch.float32, 8)):
+        g = torch.randn((N, N, 2), dtype=dt, device="cuda", generator=gen)
+        out = torch.empty_like(g)
+
+        def step(g=g, out=out):
+            ctx.fft(g, out=out, scale=1.0 / (N * N), taper=taper)
+        print(f"# image {name}: timing {args.steps} steps", file=sys.stderr, flush=True)
+        wall, per, kern = time_variant(torch, ctx, step, max(args.warmup, 1), args.steps)
+        ms = wall * 1e3 / args.steps
+        kms = float(np.mean(kern))
+        results[name] = {"ms_per_step": ms, "images_per_s": 1e3 / ms, "p50_ms": pct(per, 50),
+                         "p99_ms": pct(per, 99), "transform_ms_avg": kms,
+                         "algorithmic_gbs": 2 * N * N * esz / (kms * 1e-3) / 1e9}
+        print(f"# image {name}: {results[name]}", file=sys.stderr, flush=True)
+        del g, out
+    R = results["fp64"]
+    out_line = {
+        "metric": f"grid to image transforms/s at {N}², complex128",
+        "value": R["images_per_s"], "unit": "images/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": R["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"centred 2-D FFT of a {N}x{N} complex128 grid with taper "
+                               f"correction and scale", "variant": "fp64"},
+        "variants": results,
+        "roofline": {"bound": "hbm", "achieved": R["algorithmic_gbs"], "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": R["algorithmic_gbs"] / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "pre pass + hipFFT + post pass",
+                     "bytes_per_launch": 2 * N * N * 16},
+    }
+    print(json.dumps(out_line), flush=True)
+
+
 def main_copy(args, torch):
     """Plain device-to-device copy of 1 GiB: the calibration kernel for the
     HBM byte counters (it reads and writes exactly 2^30 bytes per launch)."""
@@ -429,11 +472,12 @@ def main():
     ap.add_argument("--stage-min", type=int, default=0, help="tiled: staging threshold")
     ap.add_argument("--wg-per-cu", type=int, default=0, help="tiled: workgroups per CU")
     ap.add_argument("--dist", default="clustered", choices=["clustered", "uniform"])
-    ap.add_argument("--mode", default="grid", choices=["grid", "batch", "serve", "copy", "degrid"],
+    ap.add_argument("--mode", default="grid", choices=["grid", "batch", "serve", "copy", "degrid", "image"],
                     help="grid: one 8192^2 job (default); batch: many jobs in one persistent "
                          "launch; serve: Poisson request stream through the serving loop; "
                          "copy: 1 GiB device copy (byte-counter calibration); degrid: the "
-                         "adjoint (grid -> visibilities) of the grid workload")
+                         "adjoint (grid -> visibilities) of the grid workload; image: the "
+                         "grid -> image FFT")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--jobs", type=int, default=64, help="batch/serve: job ids")
     ap.add_argument("--large-jobs", type=int, default=4, help="serve: 8192^2 job ids")
@@ -460,6 +504,8 @@ def main():
         return main_copy(args, torch)
     if args.mode == "degrid":
         return main_degrid(args, torch, migrid)
+    if args.mode == "image":
+        return main_image(args, torch, migrid)
     N, S, npts = args.N, args.S, args.npts
     t0 = time.perf_counter()
     uv, vals = gen_clustered(npts, N, S) if args.dist == "clustered" else gen_uniform(npts, N, S)

@@ -258,6 +258,23 @@ int migrid_ctx_grid_indexed(migrid_ctx* ctx, const migrid_job* job, int32_t ngcf
 int migrid_ctx_degrid(migrid_ctx* ctx, const migrid_job* job, void* vals_out,
                       const migrid_opts* opts, void* stream);
 
+/* Grid <-> image: the centred, unnormalised 2-D FFT (no counterpart in the
+ * reference, whose path ends at the grid).  direction MIGRID_FFT_TO_IMAGE:
+ *   img[y][x] = scale / (taper[x] taper[y]) *
+ *     sum_{v,u} grid[v][u] exp(+2 pi i ((u - N/2)(x - N/2) + (v - N/2)(y - N/2)) / N)
+ * i.e. numpy's fftshift(ifft2(ifftshift(grid))) * N^2 * scale / taper;
+ * MIGRID_FFT_TO_GRID is its adjoint (the input divided by the taper, then the
+ * exp(-...) transform, times scale).  taper: NULL or N device doubles (the
+ * separable grid correction).  in: N x N complex of `type` (interleaved, or
+ * planar (re plane, im plane) with layout PLANAR); out: N x N interleaved
+ * complex, may equal an interleaved `in`.  N even.  One hipFFT plan is cached
+ * per context.  Asynchronous on stream. */
+#define MIGRID_FFT_TO_IMAGE 1
+#define MIGRID_FFT_TO_GRID (-1)
+int migrid_ctx_fft(migrid_ctx* ctx, int32_t type, int32_t layout, const void* in, void* out,
+                   int64_t img_dim, int32_t direction, double scale, const double* taper,
+                   void* stream);
+
 /* The reference's UVW text input (grid.cu:338-366): lines
  * "%d,%d,%d: %f, %f, %f" (three integers, u, v, w), u and v scaled to the
  * grid as the reference does, x = (double)(float)(u * (float)img_dim) / 2048.

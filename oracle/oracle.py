@@ -274,6 +274,45 @@ def uvw_to_pixels(uvw, freq, N, cell, nplanes=0, w_max=1.0):
     return uv, g
 
 
+# ---- grid <-> image (no reference counterpart; numpy's FFT) -----------------
+
+def _taper2d(taper, N):
+    if taper is None:
+        return 1.0
+    t = np.asarray(taper, np.float64).reshape(N)
+    return t[:, None] * t[None, :]
+
+
+def grid_to_image(grid, scale=1.0, taper=None):
+    """Centred unnormalised transform of a complex (N, N) grid (rows v, columns
+    u) to the image plane:
+      img[y][x] = scale / (taper[x] taper[y]) *
+                  sum grid[v][u] exp(+2 pi i ((u - N/2)(x - N/2) + (v - N/2)(y - N/2)) / N)
+    = fftshift(ifft2(ifftshift(grid))) * N^2 * scale / taper2d (even N)."""
+    g = np.asarray(grid, np.complex128)
+    N = g.shape[0]
+    img = np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(g))) * (N * N)
+    return img * scale / _taper2d(taper, N)
+
+
+def image_to_grid(img, scale=1.0, taper=None):
+    """Adjoint of grid_to_image: the image divided by the taper, then the
+    exp(-...) centred transform, times scale."""
+    a = np.asarray(img, np.complex128)
+    N = a.shape[0]
+    return np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(a / _taper2d(taper, N)))) * scale
+
+
+def centred_dft_direct(a, sign):
+    """The centred 2-D DFT by its definition, O(N^4): tiny N only (pins the
+    shift conventions of grid_to_image / image_to_grid)."""
+    a = np.asarray(a, np.complex128)
+    N = a.shape[0]
+    k = np.arange(N) - N // 2
+    E = np.exp(sign * 2j * np.pi * np.outer(k, k) / N)  # E[x, u]
+    return E @ a @ E.T
+
+
 # ---- error metrics ----------------------------------------------------------
 
 def rel_errors(got, ref):

@@ -369,6 +369,7 @@ int migrid_ctx_destroy(migrid_ctx* ctx)
     ctx->queue.release();
     ctx->misc.release();
     ctx->prof.release();
+    ctx->fft.release();
     delete ctx;
     return MIGRID_OK;
 }

@@ -82,6 +82,14 @@ struct Profiler {
     void release();
 };
 
+// Cached 2-D FFT plan of a context (k_image.hip).
+struct FftPlan {
+    void* handle = nullptr;  // hipfftHandle
+    int64_t n = 0;
+    int type = 0;
+    void release();
+};
+
 }  // namespace migrid
 
 struct migrid_ctx {
@@ -91,4 +99,5 @@ struct migrid_ctx {
     migrid::Scratch records;  // packed per-visibility records
     migrid::Scratch queue;    // work queue of the tiled gridder
     migrid::Scratch misc;
+    migrid::FftPlan fft;
 };

@@ -25,6 +25,7 @@ LAYOUT_DEFAULT, LAYOUT_INTERLEAVED, LAYOUT_PLANAR = 0, 1, 2
 GCF_TABLE, GCF_COMPAT, GCF_ANALYTIC = 0, 1, 2
 ARITH_NATIVE, ARITH_F32, ARITH_F32_BF16GCF = 0, 1, 2
 F64, F32 = 0, 1
+FFT_TO_IMAGE, FFT_TO_GRID = 1, -1
 
 STRATEGIES = {"scatter": SCATTER, "gather": GATHER, "window": WINDOW, "tiled": TILED}
 
@@ -104,7 +105,7 @@ EXPORTS = [
     "migrid_server_wait", "migrid_server_read_grid", "migrid_server_drop_job",
     "migrid_server_get_stats", "migrid_server_destroy",
     "migrid_uvw_to_pixels", "migrid_ctx_grid_indexed", "migrid_read_uvw_text",
-    "migrid_ctx_degrid",
+    "migrid_ctx_degrid", "migrid_ctx_fft",
 ]
 
 _lib = None
@@ -167,6 +168,7 @@ def lib():
     L.migrid_ctx_grid_indexed.argtypes = [vp, ctypes.POINTER(Job), i32, vp, ctypes.POINTER(Opts), vp]
     L.migrid_read_uvw_text.argtypes = [ctypes.c_char_p, i64, i64, vp, vp, ctypes.POINTER(i64)]
     L.migrid_ctx_degrid.argtypes = [vp, ctypes.POINTER(Job), vp, ctypes.POINTER(Opts), vp]
+    L.migrid_ctx_fft.argtypes = [vp, i32, i32, vp, vp, i64, i32, ctypes.c_double, vp, vp]
     _lib = L
     return L
 
@@ -344,6 +346,29 @@ class Context:
         o = make_opts(**kw)
         s = ctypes.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
         check(lib().migrid_ctx_degrid(self.h, ctypes.byref(j), out.data_ptr(), ctypes.byref(o), s))
+        return out
+
+    def fft(self, x, out=None, direction=FFT_TO_IMAGE, scale=1.0, taper=None, planar=False,
+            stream=None):
+        """Centred unnormalised 2-D FFT between grid and image: FFT_TO_IMAGE
+        gives fftshift(ifft2(ifftshift(x))) * N^2 * scale / (taper[x] taper[y]),
+        FFT_TO_GRID its adjoint.  x: (N,N,2) tensor of float64/float32, or
+        (2,N,N) with planar=True; taper: None or (N,) float64 device tensor.
+        Returns the (N,N,2) result (out may be x itself when interleaved)."""
+        import torch
+        N = x.shape[1] if planar else x.shape[0]
+        if out is None:
+            out = torch.empty((N, N, 2), dtype=x.dtype, device=x.device)
+        for t in (x, out) + ((taper,) if taper is not None else ()):
+            if not t.is_contiguous() or not t.is_cuda:
+                raise MigridError(-1, "device tensors must be contiguous and on the GPU")
+        if taper is not None and (taper.dtype != torch.float64 or taper.numel() != N):
+            raise MigridError(-1, "taper must be N float64 values")
+        typ = F64 if x.dtype == torch.float64 else F32
+        s = ctypes.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
+        check(lib().migrid_ctx_fft(self.h, typ, LAYOUT_PLANAR if planar else LAYOUT_INTERLEAVED,
+                                   x.data_ptr(), out.data_ptr(), N, direction, float(scale),
+                                   taper.data_ptr() if taper is not None else None, s))
         return out
 
     def grid_indexed(self, grid, uv, vals, gcfs, gcf_index, N, S, stream=None, **kw):

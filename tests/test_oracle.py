@@ -348,3 +348,40 @@ def test_degrid_single_pixel_grid():
         c, r = 17 - (mx - S // 2 + 1), 13 - (my - S // 2 + 1)
         want = G[k, r, c, 0] - 1j * G[k, r, c, 1] if (0 <= c < S and 0 <= r < S) else 0
         assert got[n, 0] + 1j * got[n, 1] == want
+
+
+@pytest.mark.parametrize("N", [2, 6, 8])
+def test_centred_transforms_match_the_definition(N):
+    rng = np.random.default_rng(N)
+    g = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    assert np.allclose(O.grid_to_image(g), O.centred_dft_direct(g, +1), rtol=0, atol=1e-12)
+    assert np.allclose(O.image_to_grid(g), O.centred_dft_direct(g, -1), rtol=0, atol=1e-12)
+    t = 0.5 + rng.random(N)
+    want = O.centred_dft_direct(g, +1) * 2.5 / np.outer(t, t)
+    assert np.allclose(O.grid_to_image(g, 2.5, t), want, rtol=0, atol=1e-11)
+
+
+def test_centred_transforms_are_adjoint():
+    rng = np.random.default_rng(3)
+    N = 16
+    g = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    h = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    t = 0.5 + rng.random(N)
+    lhs = np.vdot(h, O.grid_to_image(g, 0.3, t))
+    rhs = np.vdot(O.image_to_grid(h, 0.3, t), g)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+@pytest.mark.parametrize("N", [8, 10, 64])
+def test_checkerboard_centring_for_even_sizes(N):
+    """The device's centring: s . FFT(s . g) with s = (-1)^(x+y) equals the
+    shifted transform for even N, and not for odd N."""
+    rng = np.random.default_rng(N)
+    g = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    s = (-1.0) ** np.add.outer(np.arange(N), np.arange(N))
+    assert np.allclose(s * np.fft.ifft2(s * g) * N * N, O.grid_to_image(g), rtol=0, atol=1e-10)
+    assert np.allclose(s * np.fft.fft2(s * g), O.image_to_grid(g), rtol=0, atol=1e-10)
+    M = N + 1
+    h = rng.normal(size=(M, M)) + 0j
+    s = (-1.0) ** np.add.outer(np.arange(M), np.arange(M))
+    assert not np.allclose(s * np.fft.ifft2(s * h) * M * M, O.centred_dft_direct(h, +1))
