@@ -29,6 +29,18 @@
 namespace migrid {
 namespace {
 
+// The automatic tile edge: the largest of 64, 32, 16 whose staged tile (with
+// its S - 1 halo, gsz bytes a pixel) lets two workgroups share a CU's 160 KB
+// of LDS, else the largest that fits one; 0 if none fits.
+constexpr int auto_tile(int S, int gsz)
+{
+    for (int t = 64; t >= 16; t /= 2)
+        if ((t + S - 1) * (t + S - 1) * gsz <= 80 * 1024) return t;
+    for (int t = 64; t >= 16; t /= 2)
+        if ((t + S - 1) * (t + S - 1) * gsz <= 160 * 1024) return t;
+    return 0;
+}
+
 struct DegridItem {
     int32_t tile, lo, hi, pad;  // sorted positions [lo, hi) of one tile
 };
@@ -50,14 +62,32 @@ struct C2 {
     T re, im;
 };
 
-__device__ __forceinline__ double wave_sum(double x)
+// Sum over the 64 lanes of a wave, left in lane 63, by DPP moves (no LDS):
+// pairs, quads (quad_perm), halves of a row (row_half_mirror), rows
+// (row_mirror), then row 0 into row 1 and row 2 into row 3 (row_bcast:15),
+// then rows 0-1 into rows 2-3 (row_bcast:31).  Every lane must be active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f(float x)
 {
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    return x;
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROWS, 0xf, false));
 }
-__device__ __forceinline__ float wave_sum(float x)
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f(double x)
 {
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    const int64_t b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
+    return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
+}
+template <typename T>
+__device__ __forceinline__ T wave_sum63(T x)
+{
+    x += dpp_f<0xb1, 0xf>(x);   // quad_perm [1,0,3,2]
+    x += dpp_f<0x4e, 0xf>(x);   // quad_perm [2,3,0,1]
+    x += dpp_f<0x141, 0xf>(x);  // row_half_mirror
+    x += dpp_f<0x140, 0xf>(x);  // row_mirror
+    x += dpp_f<0x142, 0xa>(x);  // row_bcast:15 into rows 1, 3
+    x += dpp_f<0x143, 0xc>(x);  // row_bcast:31 into rows 2, 3
     return x;
 }
 
@@ -156,21 +186,24 @@ __device__ __forceinline__ void degrid_rows(const G* __restrict__ gk, const GS* 
 {
     const int step = U * R;
     if (S % step == 0) {  // every active lane has S / R rows, in whole blocks
+        const int nb = S / step;  // the same for every lane
+        const G* __restrict__ gp = gk + rr * S;
+        const GS* tp = tile + rr * W;
         G gn[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) gn[u] = gk[(rr + u * R) * S];
-        for (int r0 = rr; r0 < S; r0 += step) {
+        for (int u = 0; u < U; ++u) gn[u] = gp[u * R * S];
+        for (int b = 0; b < nb; ++b) {
             G g[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) g[u] = gn[u];
-            if (r0 + step < S)
+            if (b + 1 < nb)
 #pragma unroll
-                for (int u = 0; u < U; ++u) gn[u] = gk[(r0 + step + u * R) * S];
+                for (int u = 0; u < U; ++u) gn[u] = gp[((b + 1) * step + u * R) * S];
             GS x[U][VV];
 #pragma unroll
             for (int u = 0; u < U; ++u)
 #pragma unroll
-                for (int i = 0; i < VV; ++i) x[u][i] = tile[off[i] + (r0 + u * R) * W];
+                for (int i = 0; i < VV; ++i) x[u][i] = tp[off[i] + (b * step + u * R) * W];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const TA gr = (TA)g[u].x, gi = (TA)g[u].y;
@@ -202,14 +235,16 @@ __device__ __forceinline__ void degrid_rows(const G* __restrict__ gk, const GS* 
 }
 
 // G: coefficient pair type, GS: staged grid pair type, TA: accumulation,
-// V: visibilities per wave pass.
-template <int V, typename G, typename GS, typename TA, typename TI, typename TO>
+// V: visibilities per wave pass; SC, WC: the support and the staged tile's
+// edge as compile-time constants (0: read from the arguments), which turns
+// the row loop's addressing into immediate offsets.
+template <int V, int SC, int WC, typename G, typename GS, typename TA, typename TI, typename TO>
 __global__ __launch_bounds__(1024) void k_degrid(DegridArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     GS* tile = (GS*)smem_raw;
-    const int N = a.N, S = a.S, T = a.T;
-    const int W = T + S - 1;
+    const int N = a.N, S = SC ? SC : a.S, T = a.T;
+    const int W = WC ? WC : T + S - 1;
     const TO* __restrict__ grid = (const TO*)a.grid;
     const int64_t NN = (int64_t)N * N;
     const int lane = threadIdx.x & 63;
@@ -222,22 +257,34 @@ __global__ __launch_bounds__(1024) void k_degrid(DegridArgs a)
     TI* __restrict__ out = (TI*)a.out;
     const int total = *a.nitems;
     constexpr int U = sizeof(TA) == 8 ? 2 : 4;  // rows per block
+    // staged pixels a thread loads per pass: all of them for a known edge,
+    // at most 5 double pairs (registers)
+    constexpr int kAll = WC ? (WC * WC + 1023) / 1024 : 4;
+    constexpr int kStagePer = sizeof(GS) == 16 && kAll > 5 ? 5 : kAll;
     for (int it = blockIdx.x; it < total; it += gridDim.x) {
         const DegridItem item = a.items[it];
         const int tx = item.tile % a.ntx, ty = item.tile / a.ntx;
         const int X0 = tx * T - (S - 1), Y0 = ty * T - (S - 1);
-        for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
-            const int yy = e / W, xx = e - yy * W;
-            const int y = Y0 + yy, x = X0 + xx;
-            GS g;
-            g.re = 0;
-            g.im = 0;
-            if (x >= 0 && y >= 0 && x < N && y < N) {
-                const int64_t p = (int64_t)y * N + x;
-                g.re = (decltype(g.re))(a.planar ? grid[p] : grid[2 * p]);
-                g.im = (decltype(g.im))(a.planar ? grid[NN + p] : grid[2 * p + 1]);
+        // all of a thread's loads in flight before its stores (one memory
+        // latency per pass rather than one per pixel)
+        for (int e0 = threadIdx.x; e0 < W * W; e0 += kStagePer * 1024) {
+            GS g[kStagePer];
+#pragma unroll
+            for (int p = 0; p < kStagePer; ++p) {
+                const int e = e0 + p * 1024;
+                const int yy = e / W, xx = e - yy * W;
+                const int y = Y0 + yy, x = X0 + xx;
+                g[p].re = 0;
+                g[p].im = 0;
+                if (e < W * W && x >= 0 && y >= 0 && x < N && y < N) {
+                    const int64_t q = (int64_t)y * N + x;
+                    g[p].re = (decltype(g[p].re))(a.planar ? grid[q] : grid[2 * q]);
+                    g[p].im = (decltype(g[p].im))(a.planar ? grid[NN + q] : grid[2 * q + 1]);
+                }
             }
-            tile[e] = g;
+#pragma unroll
+            for (int p = 0; p < kStagePer; ++p)
+                if (e0 + p * 1024 < W * W) tile[e0 + p * 1024] = g[p];
         }
         __syncthreads();
         // this wave's contiguous share of the item, in groups of V
@@ -273,10 +320,10 @@ __global__ __launch_bounds__(1024) void k_degrid(DegridArgs a)
             }
 #pragma unroll
             for (int i = 0; i < V; ++i) {
-                ar[i] = wave_sum(ar[i]);
-                ai[i] = wave_sum(ai[i]);
+                ar[i] = wave_sum63(ar[i]);
+                ai[i] = wave_sum63(ai[i]);
             }
-            if (lane == 0)
+            if (lane == 63)
 #pragma unroll
                 for (int i = 0; i < V; ++i)
                     if (i < nv) {
@@ -309,10 +356,10 @@ int blocks_for(int64_t n)
 
 size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-template <int V, typename G, typename GS, typename TA, typename TI, typename TO>
+template <int V, int SC, int WC, typename G, typename GS, typename TA, typename TI, typename TO>
 int run_degrid_v(const DegridArgs& a, size_t lds, int wg_per_cu, hipStream_t s)
 {
-    auto kern = k_degrid<V, G, GS, TA, TI, TO>;
+    auto kern = k_degrid<V, SC, WC, G, GS, TA, TI, TO>;
     MG_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
     int per_cu = 0;
@@ -327,13 +374,30 @@ int run_degrid_v(const DegridArgs& a, size_t lds, int wg_per_cu, hipStream_t s)
     return MIGRID_OK;
 }
 
+// The support-specialised kernels (S = 16, 32, 64 with the automatic tile
+// edge) or the general one.
+template <int V, typename G, typename GS, typename TA, typename TI, typename TO>
+int run_degrid_s(const DegridArgs& a, bool special, size_t lds, int wpc, hipStream_t s)
+{
+    constexpr int g = (int)sizeof(GS);
+    if (special) {
+        switch (a.S) {
+        case 16: return run_degrid_v<V, 16, auto_tile(16, g) + 15, G, GS, TA, TI, TO>(a, lds, wpc, s);
+        case 32: return run_degrid_v<V, 32, auto_tile(32, g) + 31, G, GS, TA, TI, TO>(a, lds, wpc, s);
+        case 64: return run_degrid_v<V, 64, auto_tile(64, g) + 63, G, GS, TA, TI, TO>(a, lds, wpc, s);
+        default: break;
+        }
+    }
+    return run_degrid_v<V, 0, 0, G, GS, TA, TI, TO>(a, lds, wpc, s);
+}
+
 template <typename G, typename GS, typename TA, typename TI, typename TO>
-int run_degrid(const DegridArgs& a, int V, size_t lds, int wg_per_cu, hipStream_t s)
+int run_degrid(const DegridArgs& a, int V, bool special, size_t lds, int wpc, hipStream_t s)
 {
     switch (V) {
-    case 1: return run_degrid_v<1, G, GS, TA, TI, TO>(a, lds, wg_per_cu, s);
-    case 2: return run_degrid_v<2, G, GS, TA, TI, TO>(a, lds, wg_per_cu, s);
-    default: return run_degrid_v<4, G, GS, TA, TI, TO>(a, lds, wg_per_cu, s);
+    case 1: return run_degrid_s<1, G, GS, TA, TI, TO>(a, special, lds, wpc, s);
+    case 2: return run_degrid_s<2, G, GS, TA, TI, TO>(a, special, lds, wpc, s);
+    default: return run_degrid_s<4, G, GS, TA, TI, TO>(a, special, lds, wpc, s);
     }
 }
 
@@ -374,19 +438,14 @@ extern "C" int migrid_ctx_degrid(migrid_ctx* ctx, const migrid_job* job, void* v
     const bool acc64 = (!f32) || (!in64 && out64);
     const size_t gsz = acc64 ? 16 : 8;
     auto tile_lds = [&](int t) { return (size_t)(t + S - 1) * (t + S - 1) * gsz; };
-    // the largest tile edge that lets two workgroups share a CU, else the
-    // largest that fits one
-    int T = 0;
+    int T = auto_tile(S, (int)gsz);
     if (o.tile > 0) {
         if (o.tile != 16 && o.tile != 32 && o.tile != 64)
             return fail(MIGRID_EINVAL, "degridding tile %d (16, 32 or 64)", o.tile);
         T = o.tile;
-    } else {
-        for (int t = 64; t >= 16 && !T; t /= 2)
-            if (tile_lds(t) <= 80 * 1024) T = t;
-        for (int t = 64; t >= 16 && !T; t /= 2)
-            if (tile_lds(t) <= 160 * 1024) T = t;
     }
+    // the specialised kernels assume the automatic tile edge
+    const bool special = o.tile <= 0 && (S == 16 || S == 32 || S == 64);
     if (!T || tile_lds(T) > 160 * 1024)
         return fail(MIGRID_ENOTSUP, "degridding tile does not fit in LDS");
     const size_t lds = tile_lds(T);
@@ -478,13 +537,13 @@ extern "C" int migrid_ctx_degrid(migrid_ctx* ctx, const migrid_job* job, void* v
     MG_TRY(ctx->prof.begin(s));
     int rc;
     if (!f32)  // double visibilities, double grid, double products
-        rc = run_degrid<double2, C2<double>, double, double, double>(a, V, lds, wpc, s);
+        rc = run_degrid<double2, C2<double>, double, double, double>(a, V, special, lds, wpc, s);
     else if (in64)  // double visibilities, FP32 tiles
-        rc = out64 ? run_degrid<float2, C2<float>, float, double, double>(a, V, lds, wpc, s)
-                   : run_degrid<float2, C2<float>, float, double, float>(a, V, lds, wpc, s);
+        rc = out64 ? run_degrid<float2, C2<float>, float, double, double>(a, V, special, lds, wpc, s)
+                   : run_degrid<float2, C2<float>, float, double, float>(a, V, special, lds, wpc, s);
     else  // float visibilities: sums in the grid's type
-        rc = out64 ? run_degrid<float2, C2<double>, double, float, double>(a, V, lds, wpc, s)
-                   : run_degrid<float2, C2<float>, float, float, float>(a, V, lds, wpc, s);
+        rc = out64 ? run_degrid<float2, C2<double>, double, float, double>(a, V, special, lds, wpc, s)
+                   : run_degrid<float2, C2<float>, float, float, float>(a, V, special, lds, wpc, s);
     MG_TRY(rc);
     MG_TRY(ctx->prof.end(s));
     return MIGRID_OK;
