@@ -185,8 +185,8 @@ def main_batch(args, torch, migrid):
     t0 = time.perf_counter()
     best, best_ms = migrid.autotune(ctx, jobs, arith=migrid.ARITH_F32)
     tune_s = time.perf_counter() - t0
-    print(f"# autotune: tile {best.tile} vis/item {best.vis_per_wave} {best_ms:.2f} ms "
-          f"({tune_s:.1f} s)", file=sys.stderr, flush=True)
+    print(f"# autotune: tile {best.tile} vis/item {best.vis_per_wave} wg/CU {best.wg_per_cu} "
+          f"stage_min {best.stage_min} {best_ms:.2f} ms ({tune_s:.1f} s)", file=sys.stderr, flush=True)
     b = migrid.Batch(ctx, jobs, opts=best)
     stream = torch.cuda.Stream()  # graph capture needs a non-default stream
     for _ in range(args.warmup):
@@ -222,7 +222,8 @@ def main_batch(args, torch, migrid):
                                "{1024,2048,4096}^2, supports {8,16,32,64}, clustered uv, "
                                "FP32 tiles into FP64 grids", "seed": args.seed, "npts_total": total},
         "p50_ms": pct(per, 50), "p99_ms": pct(per, 99),
-        "autotune": {"tile": best.tile, "vis_per_item": best.vis_per_wave, "ms": best_ms,
+        "autotune": {"tile": best.tile, "vis_per_item": best.vis_per_wave,
+                     "wg_per_cu": best.wg_per_cu, "stage_min": best.stage_min, "ms": best_ms,
                      "seconds": tune_s},
         "fp64_plain_one_launch_per_job": {"ms": plain_ms,
                                           "mvis_per_s": total / (plain_ms * 1e-3) / 1e6},
@@ -555,6 +556,7 @@ def main():
                    "rel_rms_vs_fp64_plain": float(torch.sqrt((d * d).sum() / (m * m).sum()))}
         ms = wall * 1e3 / args.steps
         results[name] = {"ms_per_step": ms, "mvis_per_s": npts / (ms * 1e-3) / 1e6,
+                         "grid_point_updates_per_s": npts * S * S / (ms * 1e-3),
                          "p50_ms": pct(per, 50), "p99_ms": pct(per, 99),
                          "kernel_ms_avg": float(np.mean(kern)) if kern else None,
                          "kernel_launches": len(kern), "err": err}
@@ -590,6 +592,7 @@ def main():
                    "variant": head, "grid": N, "support": S, "npts": npts},
         "p50_ms": R["p50_ms"],
         "p99_ms": R["p99_ms"],
+        "grid_point_updates_per_s": R["mvis_per_s"] * 1e6 * S * S,
         "variants": results,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak if achieved else None, "traffic": traffic,

@@ -209,7 +209,9 @@ int migrid_batch_destroy(migrid_batch* batch);
 
 /* Autotuner: times tile edge {16, 32, 64} x visibilities per wavefront
  * {64, 128, 256} (a work item holds 64 x that many visibilities) on the given
- * jobs, returns the fastest options in *best (and its time in *best_ms), and
+ * jobs, then the LDS split around the fastest: 1 or 2 workgroups per CU
+ * (opts.wg_per_cu) and the staging threshold (opts.stage_min 2 or 16, against
+ * the default 4); returns the fastest options in *best (and its time in *best_ms), and
  * caches the choice per shape class (set of supports and grid sizes,
  * arithmetic, types, visibility-count decade) for the life of the process.
  * Synchronous. */

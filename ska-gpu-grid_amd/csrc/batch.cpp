@@ -241,6 +241,23 @@ int migrid_autotune(migrid_ctx* ctx, const migrid_job* jobs, int njobs, const mi
     hipEvent_t e0, e1;
     MG_HIP(hipEventCreate(&e0));
     MG_HIP(hipEventCreate(&e1));
+    // one timed run (after a warm-up) of the batch built with options c
+    auto trial = [&](const migrid_opts& c, float* ms) -> int {
+        migrid_batch* b = nullptr;
+        int r = migrid_batch_create(ctx, jobs, njobs, &c, &b);
+        if (r != MIGRID_OK) return r;
+        r = migrid_batch_run(b, s);  // warm-up
+        if (r == MIGRID_OK) {
+            hipError_t e = hipEventRecord(e0, s);
+            if (e == hipSuccess) r = migrid_batch_run(b, s);
+            if (e == hipSuccess && r == MIGRID_OK) e = hipEventRecord(e1, s);
+            if (e == hipSuccess && r == MIGRID_OK) e = hipEventSynchronize(e1);
+            if (e == hipSuccess && r == MIGRID_OK) e = hipEventElapsedTime(ms, e0, e1);
+            if (e != hipSuccess && r == MIGRID_OK) r = fail_hip(e, "autotune timing", __FILE__, __LINE__);
+        }
+        migrid_batch_destroy(b);
+        return r;
+    };
     const int tiles[] = {16, 32, 64};
     const int vpws[] = {64, 128, 256};  // visibilities per wavefront; an item holds 64 x that
     double bt = 1e30;
@@ -251,21 +268,8 @@ int migrid_autotune(migrid_ctx* ctx, const migrid_job* jobs, int njobs, const mi
             migrid_opts c = o;
             c.tile = t;
             c.vis_per_wave = v * 64;
-            migrid_batch* b = nullptr;
-            rc = migrid_batch_create(ctx, jobs, njobs, &c, &b);
-            if (rc != MIGRID_OK) break;
-            rc = migrid_batch_run(b, s);  // warm-up
             float ms = 0.f;
-            if (rc == MIGRID_OK) {
-                hipError_t e = hipEventRecord(e0, s);
-                if (e == hipSuccess) rc = migrid_batch_run(b, s);
-                if (e == hipSuccess && rc == MIGRID_OK) e = hipEventRecord(e1, s);
-                if (e == hipSuccess && rc == MIGRID_OK) e = hipEventSynchronize(e1);
-                if (e == hipSuccess && rc == MIGRID_OK) e = hipEventElapsedTime(&ms, e0, e1);
-                if (e != hipSuccess && rc == MIGRID_OK)
-                    rc = fail_hip(e, "autotune timing", __FILE__, __LINE__);
-            }
-            migrid_batch_destroy(b);
+            rc = trial(c, &ms);
             if (rc != MIGRID_OK) break;
             if (ms < bt) {
                 bt = ms;
@@ -273,6 +277,36 @@ int migrid_autotune(migrid_ctx* ctx, const migrid_job* jobs, int njobs, const mi
             }
         }
         if (rc != MIGRID_OK) break;
+    }
+    // then the LDS split around the best shape: one or two workgroups sharing
+    // a CU's LDS, and the staging threshold (a sub-kernel with fewer
+    // visibilities for a tile is read from HBM rather than staged in LDS)
+    if (rc == MIGRID_OK) {
+        const migrid_opts shape = bo;
+        for (int w : {1, 2}) {
+            migrid_opts c = shape;
+            c.wg_per_cu = w;
+            float ms = 0.f;
+            rc = trial(c, &ms);
+            if (rc != MIGRID_OK) break;
+            if (ms < bt) {
+                bt = ms;
+                bo = c;
+            }
+        }
+        const migrid_opts split = bo;
+        for (int m : {2, 16}) {
+            if (rc != MIGRID_OK) break;
+            migrid_opts c = split;
+            c.stage_min = m;
+            float ms = 0.f;
+            rc = trial(c, &ms);
+            if (rc != MIGRID_OK) break;
+            if (ms < bt) {
+                bt = ms;
+                bo = c;
+            }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -96,9 +96,11 @@ def test_autotune_picks_a_listed_configuration_and_caches():
     ctx = migrid.Context()
     best, ms = migrid.autotune(ctx, jobs, arith=migrid.ARITH_F32)
     assert best.tile in (16, 32, 64) and best.vis_per_wave in (64 * 64, 128 * 64, 256 * 64)
+    assert best.wg_per_cu in (0, 1, 2) and best.stage_min in (0, 2, 16)  # LDS split
     assert ms > 0
     best2, ms2 = migrid.autotune(ctx, jobs, arith=migrid.ARITH_F32)
-    assert (best2.tile, best2.vis_per_wave, ms2) == (best.tile, best.vis_per_wave, ms)
+    assert (best2.tile, best2.vis_per_wave, best2.wg_per_cu, best2.stage_min, ms2) == \
+        (best.tile, best.vis_per_wave, best.wg_per_cu, best.stage_min, ms)
     b = migrid.Batch(ctx, jobs, opts=best)
     b.run()
     torch().cuda.synchronize()
