@@ -30,12 +30,11 @@ namespace migrid {
 namespace {
 
 // The automatic tile edge: the largest of 64, 32, 16 whose staged tile (with
-// its S - 1 halo, gsz bytes a pixel) lets two workgroups share a CU's 160 KB
-// of LDS, else the largest that fits one; 0 if none fits.
+// its S - 1 halo, gsz bytes a pixel) fits a CU's 160 KB of LDS; 0 if none
+// fits.  Larger tiles mean fewer stagings and longer runs of one sub-kernel,
+// which outweigh a second workgroup per CU (measured at S = 64, FP32).
 constexpr int auto_tile(int S, int gsz)
 {
-    for (int t = 64; t >= 16; t /= 2)
-        if ((t + S - 1) * (t + S - 1) * gsz <= 80 * 1024) return t;
     for (int t = 64; t >= 16; t /= 2)
         if ((t + S - 1) * (t + S - 1) * gsz <= 160 * 1024) return t;
     return 0;
