@@ -345,12 +345,17 @@ __device__ __forceinline__ void flush_px(TO* grid, TO* lo, int64_t idx_re, int64
 }
 
 // ---- the owner kernel -----------------------------------------------------------
-// Zero border of a staged sub-kernel: a wave covers a 16 x 16 tile and its
-// visibility bins are 16 pixels wide, so a listed visibility can sit up to
-// 16*ceil((S+15)/16) - 1 pixels past the tile's origin; the border makes every
-// such read land in the staged kernel or in zeros.
+// Zero border of a staged sub-kernel.  A wave covers a 16 x 16 block at a
+// multiple of 16 and its visibility bins are 16 pixels wide, so a listed
+// visibility's q lies 0 .. 16*floor((S+14)/16) + 15 pixels past the block's
+// origin.  A lane at block offset o (0..15) then reads tap S - 1 + o - (q -
+// origin), which runs from S - 1 - 16*floor((S+14)/16) - 15 up to S + 14: the
+// border before the kernel (stage_border) covers the low end, 15 more rows
+// and columns after it the high end, and every such read lands in the staged
+// kernel or in zeros.  At S <= 64 the layout is at most 95 x 104 elements
+// (158 KB of double pairs).
 __host__ __device__ constexpr int stage_border(int S) { return 16 * ((S + 15 + 15) / 16) - S; }
-__host__ __device__ constexpr int stage_rows(int S) { return S + 2 * stage_border(S); }
+__host__ __device__ constexpr int stage_rows(int S) { return S + stage_border(S) + 15; }
 // Row pitch = 8 (mod 32) elements: the 8 rows of an 8 x 8 lane block fall in
 // different LDS banks for 4-, 8- and 16-byte elements.
 __host__ __device__ constexpr int stage_pitch(int S)
@@ -358,6 +363,8 @@ __host__ __device__ constexpr int stage_pitch(int S)
     return stage_rows(S) + ((8 - stage_rows(S) % 32) + 32) % 32;
 }
 constexpr int kStageMinDefault = 4;  // fewer visibilities for a k: read the table from HBM
+// LDS of a workgroup beside the kernel's static arrays (160 KB per CU)
+constexpr size_t kMaxLds = 160 * 1024 - 4096;
 
 template <typename R>
 using ConstPtr = const __attribute__((address_space(4))) R*;  // scalar-loadable
@@ -705,6 +712,9 @@ template <int WX, int WY, typename G, typename TA, typename R, typename TI,
 int run_owner(const OwnerLaunch& L, hipStream_t s)
 {
     auto kern = k_owner<WX, WY, G, TA, R, TI, TO, PLANAR, GM>;
+    if (L.lds > kMaxLds)
+        return fail(MIGRID_ENOTSUP, "tiled: the staged sub-kernel needs %zu bytes of LDS (at most %zu)",
+                    L.lds, kMaxLds);
     MG_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)L.lds));
     int per_cu = 0;
