@@ -119,13 +119,82 @@ void oracle_grid_zz_rows(double* out, const double* uv, const double* vals,
     grid_zz_band(out, uv, vals, npts, N, gcf, S, 0, 0, y_lo, y_hi);
 }
 
-/* Multi-threaded baseline: OpenMP over row bands.  Each pixel still receives
- * its contributions in ascending n, so the result is bit-identical to the
- * serial loop.  Returns the number of threads used. */
+/* Rows [y_lo, y_hi) from the listed visibilities only (ascending n), each
+ * visibility visiting just its rows inside the band. */
+static void grid_zz_list(double* out, const double* uv, const double* vals, const int64_t* list,
+                         int64_t count, int64_t N, const double* gcf, int64_t S, int64_t y_lo,
+                         int64_t y_hi)
+{
+    double* re = out;
+    double* im = out + N * N;
+    const int64_t h = S / 2;
+    for (int64_t i = 0; i < count; i++) {
+        const int64_t n = list[i];
+        int mx, my, sx, sy;
+        index_exact(uv[2 * n], &mx, &sx);
+        index_exact(uv[2 * n + 1], &my, &sy);
+        const int64_t k = (int64_t)OVERSAMPLE * sy + sx;
+        const double* g = gcf + 2 * k * S * S;
+        const double r1 = vals[2 * n], i1 = vals[2 * n + 1];
+        int64_t b_hi = y_hi - 1 - my, b_lo = y_lo - my; /* rows of the band */
+        if (b_hi > h) b_hi = h;
+        if (b_lo < -h + 1) b_lo = -h + 1;
+        for (int64_t a = h; a > -h; a--) {
+            const int64_t x = mx + a;
+            if (x < 0 || x >= N) continue;
+            for (int64_t b = b_hi; b >= b_lo; b--) {
+                const int64_t y = my + b;
+                if (y < 0 || y >= N) continue;
+                const int64_t t = (b + h - 1) * S + (a + h - 1);
+                const double r2 = g[2 * t], i2 = g[2 * t + 1];
+                re[x + N * y] += r1 * r2 - i1 * i2;
+                im[x + N * y] += r1 * i2 + r2 * i1;
+            }
+        }
+    }
+}
+
+/* Multi-threaded baseline: OpenMP over bands of 16 rows.  One serial pass
+ * lists each band's visibilities in ascending n; each thread then grids its
+ * bands from their lists.  Every pixel still receives its contributions in
+ * ascending n, so the result is bit-identical to the serial loop.  Returns
+ * the number of threads used (0 if the lists could not be allocated). */
 int oracle_grid_zz_omp(double* out, const double* uv, const double* vals, int64_t npts,
                        int64_t N, const double* gcf, int64_t S, int nthreads)
 {
     memset(out, 0, sizeof(double) * 2 * N * N);
+    const int64_t band = 16, nb = (N + band - 1) / band, h = S / 2;
+    int64_t* start = (int64_t*)calloc((size_t)nb + 1, sizeof(int64_t));
+    if (!start) return 0;
+    /* the band range of visibility n's rows [my - h + 1, my + h] on the grid */
+#define OMP_BANDS(n, b0, b1)                                                     \
+    int mx_, my_, sx_, sy_;                                                      \
+    index_exact(uv[2 * (n)], &mx_, &sx_);                                        \
+    index_exact(uv[2 * (n) + 1], &my_, &sy_);                                    \
+    int64_t r0_ = (int64_t)my_ - h + 1, r1_ = (int64_t)my_ + h;                  \
+    if (r0_ < 0) r0_ = 0;                                                        \
+    if (r1_ > N - 1) r1_ = N - 1;                                                \
+    const int64_t k_ = (int64_t)OVERSAMPLE * sy_ + sx_;                          \
+    const int64_t b0 = r0_ / band, b1 = (r0_ <= r1_ && k_ >= 0 && k_ < 64) ? r1_ / band : b0 - 1
+    for (int64_t n = 0; n < npts; n++) {
+        OMP_BANDS(n, b0, b1);
+        for (int64_t b = b0; b <= b1; b++) start[b + 1]++;
+    }
+    for (int64_t b = 0; b < nb; b++) start[b + 1] += start[b];
+    int64_t* list = (int64_t*)malloc((size_t)(start[nb] > 0 ? start[nb] : 1) * sizeof(int64_t));
+    int64_t* fill = (int64_t*)malloc((size_t)nb * sizeof(int64_t));
+    if (!list || !fill) {
+        free(start);
+        free(list);
+        free(fill);
+        return 0;
+    }
+    memcpy(fill, start, (size_t)nb * sizeof(int64_t));
+    for (int64_t n = 0; n < npts; n++) {
+        OMP_BANDS(n, b0, b1);
+        for (int64_t b = b0; b <= b1; b++) list[fill[b]++] = n;
+    }
+#undef OMP_BANDS
     int used = 1;
 #pragma omp parallel num_threads(nthreads)
     {
@@ -135,13 +204,15 @@ int oracle_grid_zz_omp(double* out, const double* uv, const double* vals, int64_
         tid = omp_get_thread_num();
 #endif
         if (tid == 0) used = (int)nt;
-        /* interleaved bands of 16 rows balance clustered inputs */
-        const int64_t band = 16;
-        for (int64_t y0 = tid * band; y0 < N; y0 += nt * band) {
-            int64_t y1 = y0 + band < N ? y0 + band : N;
-            grid_zz_band(out, uv, vals, npts, N, gcf, S, 0, 0, y0, y1);
+        /* interleaved bands balance clustered inputs */
+        for (int64_t b = tid; b < nb; b += nt) {
+            const int64_t y0 = b * band, y1 = y0 + band < N ? y0 + band : N;
+            grid_zz_list(out, uv, vals, list + start[b], start[b + 1] - start[b], N, gcf, S, y0, y1);
         }
     }
+    free(start);
+    free(list);
+    free(fill);
     return used;
 }
 

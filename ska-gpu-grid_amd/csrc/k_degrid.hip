@@ -40,6 +40,9 @@ constexpr int auto_tile(int S, int gsz)
     return 0;
 }
 
+// Threads of a degridding workgroup: 16 waves (the staging loop strides by it).
+constexpr int kDegridThreads = 1024;
+
 struct DegridItem {
     int32_t tile, lo, hi, pad;  // sorted positions [lo, hi) of one tile
 };
@@ -238,7 +241,7 @@ __device__ __forceinline__ void degrid_rows(const G* __restrict__ gk, const GS* 
 // edge as compile-time constants (0: read from the arguments), which turns
 // the row loop's addressing into immediate offsets.
 template <int V, int SC, int WC, typename G, typename GS, typename TA, typename TI, typename TO>
-__global__ __launch_bounds__(1024) void k_degrid(DegridArgs a)
+__global__ __launch_bounds__(kDegridThreads) void k_degrid(DegridArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     GS* tile = (GS*)smem_raw;
@@ -248,7 +251,7 @@ __global__ __launch_bounds__(1024) void k_degrid(DegridArgs a)
     const int64_t NN = (int64_t)N * N;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int nw = blockDim.x >> 6;
+    constexpr int nw = kDegridThreads / 64;
     const int R = 64 / S;  // rows a pass covers (S <= 64)
     const int c = lane % S, rr = lane / S;
     const bool active = rr < R;
@@ -258,7 +261,7 @@ __global__ __launch_bounds__(1024) void k_degrid(DegridArgs a)
     constexpr int U = sizeof(TA) == 8 ? 2 : 4;  // rows per block
     // staged pixels a thread loads per pass: all of them for a known edge,
     // at most 5 double pairs (registers)
-    constexpr int kAll = WC ? (WC * WC + 1023) / 1024 : 4;
+    constexpr int kAll = WC ? (WC * WC + kDegridThreads - 1) / kDegridThreads : 4;
     constexpr int kStagePer = sizeof(GS) == 16 && kAll > 5 ? 5 : kAll;
     for (int it = blockIdx.x; it < total; it += gridDim.x) {
         const DegridItem item = a.items[it];
@@ -266,11 +269,11 @@ __global__ __launch_bounds__(1024) void k_degrid(DegridArgs a)
         const int X0 = tx * T - (S - 1), Y0 = ty * T - (S - 1);
         // all of a thread's loads in flight before its stores (one memory
         // latency per pass rather than one per pixel)
-        for (int e0 = threadIdx.x; e0 < W * W; e0 += kStagePer * 1024) {
+        for (int e0 = threadIdx.x; e0 < W * W; e0 += kStagePer * kDegridThreads) {
             GS g[kStagePer];
 #pragma unroll
             for (int p = 0; p < kStagePer; ++p) {
-                const int e = e0 + p * 1024;
+                const int e = e0 + p * kDegridThreads;
                 const int yy = e / W, xx = e - yy * W;
                 const int y = Y0 + yy, x = X0 + xx;
                 g[p].re = 0;
@@ -283,7 +286,7 @@ __global__ __launch_bounds__(1024) void k_degrid(DegridArgs a)
             }
 #pragma unroll
             for (int p = 0; p < kStagePer; ++p)
-                if (e0 + p * 1024 < W * W) tile[e0 + p * 1024] = g[p];
+                if (e0 + p * kDegridThreads < W * W) tile[e0 + p * kDegridThreads] = g[p];
         }
         __syncthreads();
         // this wave's contiguous share of the item, in groups of V
@@ -362,13 +365,13 @@ int run_degrid_v(const DegridArgs& a, size_t lds, int wg_per_cu, hipStream_t s)
     MG_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));
     int per_cu = 0;
-    MG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 1024, lds));
+    MG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kDegridThreads, lds));
     if (per_cu < 1) return fail(MIGRID_ENOTSUP, "degridding kernel does not fit (lds=%zu)", lds);
     if (wg_per_cu > 0 && wg_per_cu < per_cu) per_cu = wg_per_cu;
     int dev = 0, ncu = 0;
     MG_HIP(hipGetDevice(&dev));
     MG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    hipLaunchKernelGGL(kern, dim3((unsigned)(ncu * per_cu)), dim3(1024), lds, s, a);
+    hipLaunchKernelGGL(kern, dim3((unsigned)(ncu * per_cu)), dim3(kDegridThreads), lds, s, a);
     MG_HIP(hipGetLastError());
     return MIGRID_OK;
 }

@@ -119,6 +119,12 @@ def test_row_bands_and_openmp_are_bit_identical():
     part = O.grid_zz_rows(uv, vals, N, gcf, S, 0, 100)
     O.grid_zz_rows(uv, vals, N, gcf, S, 100, N, out=part)
     np.testing.assert_array_equal(part, ref)
+    # windows crossing every edge of a grid that is not a multiple of the band
+    for S, N in ((16, 77), (64, 50), (2, 9)):
+        uv, vals = O.gen_uniform(2000, -S, N + S, seed=S + N)
+        gcf = O.config_gcf(S)
+        got, _ = O.grid_zz_omp(uv, vals, N, gcf, S, 3)
+        np.testing.assert_array_equal(got, O.grid_zz(uv, vals, N, gcf, S))
 
 
 def test_floorf_emulation_K4():
