@@ -108,18 +108,41 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t* __restri
 
 // Stable scatter of one radix pass.  Elements are visited round-major
 // (round j, thread t -> tile position j*256 + t), which is index order, and
-// each round is ranked with ballots plus an LDS prefix over the 4 waves.
+// each round is ranked with ballots plus an LDS prefix over the 4 waves.  The
+// ranked elements are placed in LDS in digit order first (the block's digit
+// counts, from k_rs_hist, give each digit's local start), then written out
+// by consecutive threads, so each digit's run of the block goes to global
+// memory as contiguous stores rather than one 4-byte store per element.
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, int64_t n, int shift, const int32_t* __restrict__ base_offs)
+    uint32_t* __restrict__ vout, int64_t n, int shift, const int32_t* __restrict__ base_offs,
+    const int32_t* __restrict__ counts)
 {
     __shared__ int wave_cnt[4][256];
     __shared__ int running[256];
     __shared__ int blk_base[256];
+    __shared__ int loc_start[256];
+    __shared__ int wave_sum[4];
+    __shared__ uint32_t sk[kRsTile];
+    __shared__ uint32_t sv[kRsTile];
     const int t = threadIdx.x;
-    const int w = t >> 6;
+    const int w = t >> 6, lane = t & 63;
     running[t] = 0;
     blk_base[t] = base_offs[(int64_t)t * gridDim.x + blockIdx.x];
+    {
+        // exclusive prefix of the block's digit counts (thread t = digit t)
+        const int c = counts[(int64_t)t * gridDim.x + blockIdx.x];
+        int x = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wave_sum[w] = x;
+        __syncthreads();
+        int off = 0;
+        for (int ww = 0; ww < w; ww++) off += wave_sum[ww];
+        loc_start[t] = off + x - c;
+    }
     const int64_t base = (int64_t)blockIdx.x * kRsTile;
     for (int j = 0; j < kRsItems; j++) {
         for (int ww = 0; ww < 4; ww++) wave_cnt[ww][t] = 0;
@@ -153,11 +176,20 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
         }
         __syncthreads();
         if (valid) {
-            const int64_t pos = (int64_t)blk_base[d] + wave_cnt[w][d] + rank;
-            kout[pos] = key;
-            vout[pos] = val;
+            const int lp = loc_start[d] + wave_cnt[w][d] + rank;  // block-local sorted position
+            sk[lp] = key;
+            sv[lp] = val;
         }
         __syncthreads();
+    }
+    const int64_t rem = n - base;
+    const int cnt_blk = rem < kRsTile ? (int)rem : kRsTile;
+    for (int e = t; e < cnt_blk; e += kRsThreads) {
+        const uint32_t key = sk[e];
+        const int d = (key >> shift) & 255;
+        const int64_t pos = (int64_t)blk_base[d] + (e - loc_start[d]);
+        kout[pos] = key;
+        vout[pos] = sv[e];
     }
 }
 
@@ -440,7 +472,7 @@ int sort_keys(int64_t E, int64_t K, void* scratch, int32_t* perm, int32_t* offse
             hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(kRsThreads), 0, s, ki, E, shift, counts);
             MG_TRY(exclusive_scan(counts, 256 * nblk, coffs, partial, false, s));
             hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(kRsThreads), 0, s, ki, vi, ko, vo, E,
-                               shift, coffs);
+                               shift, coffs, counts);
             MG_HIP(hipGetLastError());
             uint32_t* tk = ki;
             ki = ko;
