@@ -23,9 +23,10 @@ extern "C" int migrid_read_uvw_text(const char* path, int64_t img_dim, int64_t m
     if (img_dim < 1 || max_rows < 0) return fail(MIGRID_EINVAL, "img_dim/max_rows");
     FILE* f = std::fopen(path, "r");
     if (!f) return fail(MIGRID_EINVAL, "cannot open %s", path);
-    int64_t n = 0;
+    int64_t n = 0, lineno = 0;
     char line[512];
     while (n < max_rows && std::fgets(line, sizeof line, f)) {
+        lineno++;
         int a, b, c;
         float u, v, w;
         if (std::sscanf(line, "%d,%d,%d: %f, %f, %f", &a, &b, &c, &u, &v, &w) != 6) {
@@ -36,7 +37,7 @@ extern "C" int migrid_read_uvw_text(const char* path, int64_t img_dim, int64_t m
             std::fclose(f);
             *nrows_out = n;
             return fail(MIGRID_EINVAL, "%s: line %lld is not \"%%d,%%d,%%d: %%f, %%f, %%f\"", path,
-                        (long long)n + 1);
+                        (long long)lineno);
         }
         const float fi = (float)img_dim;
         uv_out[2 * n] = (double)(u * fi) / 2048.0;

@@ -41,5 +41,10 @@ def test_uvw_text_limits_and_errors(tmp_path):
     with pytest.raises(migrid.MigridError) as e:
         migrid.read_uvw_text(str(bad), 2048)
     assert "line 2" in str(e.value)
+    # the reported line counts blank lines too
+    bad.write_text("0,1,0: 1.0, 2.0, 3.0\n\n\n1,2: 1.0, 2.0\n")
+    with pytest.raises(migrid.MigridError) as e:
+        migrid.read_uvw_text(str(bad), 2048)
+    assert "line 4" in str(e.value)
     with pytest.raises(migrid.MigridError):
         migrid.read_uvw_text(str(tmp_path / "missing.dat"), 2048, max_rows=3)
