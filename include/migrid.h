@@ -330,7 +330,10 @@ int migrid_server_submit(migrid_server* server, int64_t job_id, int64_t img_dim,
                          const void* gcf, const void* uv, const void* vals, int64_t npts,
                          int32_t flags, int64_t* ticket);
 /* *done = 1 when the request has completed; returns its status then.
- * latency_ms: submit to completion as seen by the worker. */
+ * latency_ms: submit to completion as seen by the worker.  Statuses are kept
+ * for repeated reads; past 65536 completed requests the oldest (all below
+ * every pending ticket) are released down to half that, and poll/wait on a
+ * released ticket return MIGRID_EINVAL ("released") instead of blocking. */
 int migrid_server_poll(migrid_server* server, int64_t ticket, int32_t* done, double* latency_ms);
 int migrid_server_wait(migrid_server* server, int64_t ticket, double* latency_ms);
 /* Waits for every request of the job submitted so far, then copies its grid

@@ -101,6 +101,7 @@ struct migrid_server {
     std::vector<int> idle;
     std::map<int64_t, Job> jobs;
     std::map<int64_t, Done> done;
+    int64_t released_below = 1;  // statuses of tickets below this were released
     int64_t next_ticket = 1;
     size_t hbm_used = 0;   // resident grids + uploaded inputs
     size_t work_used = 0;  // admission estimates of admitted and running work
@@ -166,6 +167,27 @@ void finish(migrid_server* s, Request& r, int status)
         auto it = s->jobs.find(r.job_id);
         if (it != s->jobs.end()) it->second.inflight--;
         r.counted = false;
+    }
+}
+
+// Completed statuses kept for poll/wait.  Past kDoneKeep entries, the oldest
+// are released down to half of it -- only tickets below every pending one,
+// which have all completed, so a released ticket is never one still running.
+constexpr size_t kDoneKeep = (size_t)1 << 16;
+
+void prune_done(migrid_server* s)
+{
+    if (s->done.size() <= kDoneKeep) return;
+    int64_t min_pending = s->next_ticket;
+    for (const Request& r : s->pending) min_pending = std::min(min_pending, r.ticket);
+    for (const Request& r : s->admitted) min_pending = std::min(min_pending, r.ticket);
+    for (int b : s->busy)
+        for (const Request& r : s->slots[b].reqs) min_pending = std::min(min_pending, r.ticket);
+    while (s->done.size() > kDoneKeep / 2) {
+        auto it = s->done.begin();
+        if (it->first >= min_pending) break;
+        s->released_below = it->first + 1;
+        s->done.erase(it);
     }
 }
 
@@ -355,6 +377,7 @@ bool reap(migrid_server* s)
         sl.reqs.clear();
         s->idle.push_back(s->busy.front());
         s->busy.pop_front();
+        prune_done(s);
         any = true;
     }
     return any;
@@ -527,6 +550,9 @@ int migrid_server_poll(migrid_server* s, int64_t ticket, int32_t* done, double* 
     if (!s || !done) return fail(MIGRID_EINVAL, "NULL argument");
     std::lock_guard<std::mutex> lk(s->mu);
     if (ticket <= 0 || ticket >= s->next_ticket) return fail(MIGRID_EINVAL, "unknown ticket");
+    if (ticket < s->released_below)
+        return fail(MIGRID_EINVAL, "request %lld completed long ago; its status was released",
+                    (long long)ticket);
     auto it = s->done.find(ticket);
     *done = it != s->done.end();
     if (!*done) return MIGRID_OK;
@@ -542,7 +568,10 @@ int migrid_server_wait(migrid_server* s, int64_t ticket, double* latency_ms)
     if (!s) return fail(MIGRID_EINVAL, "server is NULL");
     std::unique_lock<std::mutex> lk(s->mu);
     if (ticket <= 0 || ticket >= s->next_ticket) return fail(MIGRID_EINVAL, "unknown ticket");
-    s->done_cv.wait(lk, [&] { return s->done.count(ticket) > 0; });
+    s->done_cv.wait(lk, [&] { return s->done.count(ticket) > 0 || ticket < s->released_below; });
+    if (ticket < s->released_below)
+        return fail(MIGRID_EINVAL, "request %lld completed long ago; its status was released",
+                    (long long)ticket);
     const Done d = s->done[ticket];
     if (latency_ms) *latency_ms = d.latency_ms;
     if (d.status != MIGRID_OK)

@@ -150,3 +150,35 @@ def test_bad_requests_fail_alone():
             srv.submit(1, uv, vals, gcf, N, 7)  # odd support: refused at submit
         srv.wait(srv.submit(1, uv, vals, gcf, N, S))
         assert O.rel_errors(srv.read_grid(1), 2 * ref(uv, vals, gcf, N, S))[0] < 1e-5
+
+
+def test_completed_statuses_are_bounded():
+    """Past 65536 completed requests the oldest statuses are released: an old
+    ticket reads as released (an error, not a hang), a recent one as done, and
+    every request still completes and adds into its grid."""
+    t = torch()
+    N, S, jobs, n = 32, 8, 64, 70000
+    g_np = O.config_gcf(S)
+    uv_np = np.array([[16.25, 15.5]])
+    v_np = np.array([[1.0, -0.5]])
+    gcf = t.from_numpy(g_np).cuda()
+    uv = t.from_numpy(uv_np).cuda()
+    vals = t.from_numpy(v_np).cuda()
+    print(f"[server] {n} one-visibility requests on {jobs} job ids", flush=True)
+    with migrid.Server() as srv:
+        tickets = [srv.submit(j % jobs, uv, vals, gcf, N, S) for j in range(n)]
+        srv.wait(tickets[-1])
+        for tk in tickets[-100:]:
+            srv.wait(tk)
+        st = srv.stats()
+        assert st["completed"] == n and st["failed"] == 0
+        with pytest.raises(migrid.MigridError, match="released"):
+            srv.poll(tickets[0])
+        with pytest.raises(migrid.MigridError, match="released"):
+            srv.wait(tickets[1])
+        assert srv.poll(tickets[-1])[0]
+        one = ref(uv_np, v_np, g_np, N, S)
+        for j in (0, jobs - 1):
+            got = srv.read_grid(j)
+            count = len(range(j, n, jobs))
+            assert np.abs(got - count * one).max() <= 1e-5 * count * np.abs(one).max()
