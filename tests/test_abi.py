@@ -105,3 +105,28 @@ def test_server_null_handles_rejected():
     assert L.migrid_server_wait(None, 1, None) == -1
     assert L.migrid_server_read_grid(None, 1, None) == -1
     assert L.migrid_server_destroy(None) == 0
+
+
+def test_new_entry_points_reject_bad_arguments_before_any_device_call():
+    L = migrid.lib()
+    d = (ctypes.c_double * 8)()
+    i = (ctypes.c_int32 * 4)()
+    # NULL contexts
+    assert L.migrid_ctx_fft(None, migrid.F64, migrid.LAYOUT_INTERLEAVED, d, d, 8,
+                            migrid.FFT_TO_IMAGE, 1.0, None, None) == -1
+    j = migrid.Job()
+    assert L.migrid_ctx_degrid(None, ctypes.byref(j), d, None, None) == -1
+    assert L.migrid_ctx_grid_indexed(None, ctypes.byref(j), 1, i, None, None) == -1
+    # uv -> pixel conversion: shapes and parameters checked first
+    assert L.migrid_uvw_to_pixels(d, 1, d, 0, 64, 1e-4, 0, 1.0, d, None, None) == -1   # nchan
+    assert L.migrid_uvw_to_pixels(d, 1, d, 1, 64, 0.0, 0, 1.0, d, None, None) == -1    # cell
+    assert L.migrid_uvw_to_pixels(d, 1, d, 1, 64, 1e-4, 4, 0.0, d, i, None) == -1      # w_max
+    assert L.migrid_uvw_to_pixels(d, 1, d, 1, 64, 1e-4, 4, 1.0, d, None, None) == -1   # no index out
+    assert L.migrid_uvw_to_pixels(None, 1, d, 1, 64, 1e-4, 0, 1.0, d, None, None) == -1  # NULL uvw
+    assert L.migrid_uvw_to_pixels(None, 0, None, 1, 64, 1e-4, 0, 1.0, None, None, None) == 0  # empty
+    # the text reader checks its arguments
+    n = ctypes.c_int64()
+    assert L.migrid_read_uvw_text(None, 64, 1, d, None, ctypes.byref(n)) == -1
+    assert L.migrid_read_uvw_text(b"/nonexistent/uvw.dat", 64, 1, d, None, ctypes.byref(n)) == -1
+    assert "cannot open" in L.migrid_last_error().decode()
+    assert L.migrid_read_uvw_text(b"x", 0, 1, d, None, ctypes.byref(n)) == -1  # img_dim
