@@ -487,7 +487,8 @@ def main():
     ap.add_argument("--S", type=int, default=64)
     ap.add_argument("--cpu-sample", type=int, default=40000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variants", default="fp64_plain,fp64_tiled,fp32_tiled,bf16_tiled")
+    ap.add_argument("--variants", default="fp64_plain,fp64_tiled,fp32_tiled,bf16_tiled,fp64_gather,"
+                                          "fp64_window,fp64_plain_computed,fp64_tiled_computed")
     ap.add_argument("--tile", type=int, default=0, help="tiled: tile edge 16/32/64 (0 auto)")
     ap.add_argument("--vpw", type=int, default=0, help="tiled: visibilities per work item")
     ap.add_argument("--stage-min", type=int, default=0, help="tiled: staging threshold")
@@ -545,15 +546,25 @@ def main():
         "fp64_tiled": dict(strategy="tiled", arith=migrid.ARITH_NATIVE),
         "fp32_tiled": dict(strategy="tiled", arith=migrid.ARITH_F32),
         "bf16_tiled": dict(strategy="tiled", arith=migrid.ARITH_F32_BF16GCF),
+        # the reference's other two strategies, FP64 (interleaved for the comparison)
+        "fp64_gather": dict(strategy="gather"),
+        "fp64_window": dict(strategy="window", layout=migrid.LAYOUT_INTERLEAVED),
+        # COMPUTE_GCF: the phase form evaluated per tap, scatter and tiled
+        "fp64_plain_computed": dict(strategy="scatter", gcf_mode=migrid.GCF_COMPAT),
+        "fp64_tiled_computed": dict(strategy="tiled", gcf_mode=migrid.GCF_COMPAT),
     }
     tune = dict(tile=args.tile, vis_per_wave=args.vpw, stage_min=args.stage_min,
                 wg_per_cu=args.wg_per_cu)
     results = {}
-    ref_grid = None
+    ref_grid = None       # the plain FP64 scatter's grid (table GCF)
+    ref_computed = None   # the plain FP64 scatter's grid (COMPUTE_GCF)
     for name in args.variants.split(","):
         kw = dict(variants[name])
-        if kw["strategy"] == "tiled":
+        if kw["strategy"] != "scatter":
             kw.update(tune)
+        # the plain scatter runs ~1 s a call: fewer steps (the headline keeps K, W)
+        plain = kw["strategy"] == "scatter"
+        warmup, steps = (min(args.warmup, 1), min(args.steps, 3)) if plain else (args.warmup, args.steps)
 
         def step(kw=kw):
             ctx.grid(grid, d_uv, d_vals, d_gcf, N, S, **kw)
@@ -563,23 +574,27 @@ def main():
             print(f"# {name}: not available ({e})", file=sys.stderr, flush=True)
             continue
         torch.cuda.synchronize()
-        print(f"# {name}: timing {args.steps} steps", file=sys.stderr, flush=True)
-        wall, per, kern = time_variant(torch, ctx, step, args.warmup, args.steps)
+        print(f"# {name}: timing {steps} steps", file=sys.stderr, flush=True)
+        wall, per, kern = time_variant(torch, ctx, step, warmup, steps)
         g = grid.clone()
-        if ref_grid is None and name == "fp64_plain":
+        computed = kw.get("gcf_mode", migrid.GCF_TABLE) == migrid.GCF_COMPAT
+        if name == "fp64_plain":
             ref_grid = g
+        if name == "fp64_plain_computed":
+            ref_computed = g
+        ref, ref_name = (ref_computed, "fp64_plain_computed") if computed else (ref_grid, "fp64_plain")
         err = None
-        if ref_grid is not None and name != "fp64_plain":
-            d = torch.linalg.vector_norm(g - ref_grid, dim=-1)
-            m = torch.linalg.vector_norm(ref_grid, dim=-1)
-            err = {"rel_max_vs_fp64_plain": float(d.max() / m.max()),
-                   "rel_rms_vs_fp64_plain": float(torch.sqrt((d * d).sum() / (m * m).sum()))}
-        ms = wall * 1e3 / args.steps
+        if ref is not None and ref is not g:
+            d = torch.linalg.vector_norm(g - ref, dim=-1)
+            m = torch.linalg.vector_norm(ref, dim=-1)
+            err = {"rel_max_vs_" + ref_name: float(d.max() / m.max()),
+                   "rel_rms_vs_" + ref_name: float(torch.sqrt((d * d).sum() / (m * m).sum()))}
+        ms = wall * 1e3 / steps
         results[name] = {"ms_per_step": ms, "mvis_per_s": npts / (ms * 1e-3) / 1e6,
                          "grid_point_updates_per_s": npts * S * S / (ms * 1e-3),
                          "p50_ms": pct(per, 50), "p99_ms": pct(per, 99),
                          "kernel_ms_avg": float(np.mean(kern)) if kern else None,
-                         "kernel_launches": len(kern), "err": err}
+                         "kernel_launches": len(kern), "steps": steps, "err": err}
         print(f"# {name}: {results[name]}", file=sys.stderr, flush=True)
         del g
     if not results:

@@ -238,6 +238,7 @@ __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs,
     uint8_t* __restrict__ split = (uint8_t*)jobs[job].split;
     const bool fresh = jobs[job].fresh != 0;
     const bool shared = jobs[job].shared != 0;
+    const bool det = jobs[job].det != 0;
     const int lane = threadIdx.x & 63;
     const int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t ntiles = (int64_t)g.ntx * g.ntx;
@@ -275,9 +276,26 @@ __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs,
     const bool start = cnt > 0 && (prev < 0 || j != jprev);
     const uint64_t m = __ballot(start);
     const int n_items = __popcll(m);
-    int base = 0;
-    if (lane == 0) base = atomicAdd(&qs->nitems, n_items);
+    int base = 0, pbase = -1;
+    if (lane == 0) {
+        base = atomicAdd(&qs->nitems, n_items);
+        if (det && n_items > 1) {  // list the split tile for the ordered reduction
+            const int rb = atomicAdd(&qs->nreduce, 1);
+            const int pb = atomicAdd(&qs->npartial, n_items);
+            const bool fits = rb < jobs[job].rcap && pb + n_items <= jobs[job].pcap;
+            if (rb < jobs[job].rcap) {
+                ReduceItem ri;
+                ri.job = job;
+                ri.tile = (int32_t)tile;
+                ri.pslot = pb;
+                ri.n = fits ? n_items : 0;
+                jobs[job].reduce[rb] = ri;
+            }
+            if (fits) pbase = pb;
+        }
+    }
     base = __shfl(base, 0, 64);
+    pbase = __shfl(pbase, 0, 64);
     if (start) {
         const int slot = base + __popcll(m & ((1ull << lane) - 1));
         const uint64_t above = lane == 63 ? 0 : (m >> (lane + 1));
@@ -287,11 +305,11 @@ __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs,
         it.klo = (int16_t)lane;
         it.khi = (int16_t)khi;
         it.job = job;
-        it.pad = 0;
+        it.pslot = pbase >= 0 ? pbase + __popcll(m & ((1ull << lane) - 1)) : -1;
         items[slot] = it;
     }
     if (lane == 0) split[tile] = n_items > 1 || shared;
-    if (fresh && n_items > 1) {
+    if (fresh && n_items > 1 && pbase < 0) {  // split tiles flushed with atomics start at zero
         zero_tile((TO*)jobs[job].grid, g.N, tx * g.T, ty * g.T, g.T, planar != 0);
         if (jobs[job].grid_lo)
             zero_tile((TO*)jobs[job].grid_lo, g.N, tx * g.T, ty * g.T, g.T, planar != 0);
@@ -470,6 +488,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
         const int klo = __builtin_amdgcn_readfirstlane(it.klo);
         const int khi = __builtin_amdgcn_readfirstlane(it.khi);
         const int job = __builtin_amdgcn_readfirstlane(it.job);
+        const int pslot = __builtin_amdgcn_readfirstlane(it.pslot);
         const TileGeom g = jobs[job].g;
         ConstPtr<R> rec = (ConstPtr<R>)(uintptr_t)jobs[job].rec;
         const G* __restrict__ gtab = (const G*)jobs[job].gcf;  // table in the staged type
@@ -651,7 +670,21 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
                 }
             }
         }
-        // flush
+        // flush: a split tile of the ordered-reduction mode writes its
+        // partial tile (T x T accumulators) to its slot
+        if (pslot >= 0) {
+            TA* part = (TA*)jobs[job].partials + (int64_t)pslot * g.T * g.T * 2;
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    const int e = (Yw - Y0 + 8 * i + ly) * g.T + (Xw - X0 + 8 * j + lx);
+                    part[2 * e] = acc[i][j].re;
+                    part[2 * e + 1] = acc[i][j].im;
+                }
+            __syncthreads();  // s_item reuse
+            continue;
+        }
 #pragma unroll
         for (int i = 0; i < 2; i++) {
             const int y = Yw + 8 * i + ly;
@@ -669,6 +702,46 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
             }
         }
         __syncthreads();  // s_item reuse
+    }
+}
+
+// Ordered reduction of split tiles (gather / window): each listed tile's
+// partial tiles are summed per pixel in item order and flushed once -- a
+// plain store into a fresh grid, else read-add-write (the tile is this
+// block's alone).  The grid is the same bit for bit on every run.
+template <typename TA, typename TO>
+__global__ __launch_bounds__(256) void k_reduce_tiles(const JobDesc* __restrict__ jobs,
+                                                      const QueueState* __restrict__ qs,
+                                                      int planar)
+{
+    const int nr = min(qs->nreduce, jobs[0].rcap);
+    const ReduceItem* __restrict__ list = jobs[0].reduce;
+    for (int r = blockIdx.x; r < nr; r += gridDim.x) {
+        const ReduceItem ri = list[r];
+        if (ri.n <= 0) continue;
+        const JobDesc& d = jobs[ri.job];
+        const TileGeom g = d.g;
+        const int tx = ri.tile % g.ntx, ty = ri.tile / g.ntx;
+        const int64_t slot = (int64_t)g.T * g.T * 2;
+        const TA* __restrict__ part = (const TA*)d.partials + (int64_t)ri.pslot * slot;
+        TO* grid = (TO*)d.grid;
+        TO* lo = (TO*)d.grid_lo;
+        const bool fresh = d.fresh != 0;
+        const int64_t NN = (int64_t)g.N * g.N;
+        for (int e = threadIdx.x; e < g.T * g.T; e += blockDim.x) {
+            const int x = tx * g.T + e % g.T, y = ty * g.T + e / g.T;
+            if (x >= g.N || y >= g.N) continue;
+            TA re = 0, im = 0;
+            for (int i = 0; i < ri.n; i++) {
+                re += part[i * slot + 2 * e];
+                im += part[i * slot + 2 * e + 1];
+            }
+            const int64_t p = (int64_t)y * g.N + x;
+            if (planar)
+                flush_px(grid, lo, p, NN + p, re, im, false, fresh);
+            else
+                flush_px(grid, lo, 2 * p, 2 * p + 1, re, im, false, fresh);
+        }
     }
 }
 
@@ -917,12 +990,6 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     off += align256((size_t)item_cap * sizeof(Item));
     p.off_qs = off;
     off += align256(sizeof(QueueState));
-    p.arena_bytes = off;
-    p.bin_bytes = bin_scratch_bytes(p.E, p.K);
-    p.item_cap = item_cap;
-    p.target = o.strategy == MIGRID_TILED ? (o.vis_per_wave > 0 ? o.vis_per_wave : 4096) : 0;
-    p.stage_min = o.stage_min;
-    p.wg_per_cu = o.wg_per_cu;
     // jobs that write one grid flush atomically, into a grid cleared beforehand
     {
         std::vector<const void*> grids;
@@ -931,6 +998,31 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         p.shared = std::adjacent_find(grids.begin(), grids.end()) != grids.end();
     }
     p.fresh = !o.accumulate && !p.shared;
+    p.target = o.vis_per_wave > 0 ? o.vis_per_wave : 4096;
+    // gather / window: split tiles are summed by an ordered reduction, not
+    // atomics.  A tile of c > target visibilities has at most
+    // ceil(c / target) < 2c / target items, so 2E / target + 1 partial tiles
+    // and E / target + 1 split tiles bound what the work-item pass can list.
+    p.det = o.strategy != MIGRID_TILED && !p.shared;
+    if (p.det) {
+        const int64_t units = (p.E + p.target - 1) / p.target;
+        const size_t tsz = (p.key.stage == kStageF64 || p.key.acc64) ? 8 : 4;  // accumulator
+        // at most 4 GiB of partial tiles; a split tile that finds no room
+        // (only with tiny work items) flushes atomically instead
+        const int64_t slot = (int64_t)p.key.T * p.key.T * 2 * (int64_t)tsz;
+        const int64_t max_slots = std::max<int64_t>(64, ((int64_t)4 << 30) / slot);
+        p.partial_cap = std::min<int64_t>(2 * units + 1, max_slots);
+        p.reduce_cap = std::min<int64_t>(units + 1, p.partial_cap);
+        p.off_partials = off;
+        off += align256((size_t)p.partial_cap * p.key.T * p.key.T * 2 * tsz);
+        p.off_reduce = off;
+        off += align256((size_t)p.reduce_cap * sizeof(ReduceItem));
+    }
+    p.arena_bytes = off;
+    p.bin_bytes = bin_scratch_bytes(p.E, p.K);
+    p.item_cap = item_cap;
+    p.stage_min = o.stage_min;
+    p.wg_per_cu = o.wg_per_cu;
     const size_t gsz = p.key.stage == kStageBf16 ? 4 : p.key.stage == kStageF32 ? 8 : 16;
     p.lds = (size_t)stage_rows(p.smax) * stage_pitch(p.smax) * gsz;
     return MIGRID_OK;
@@ -953,6 +1045,11 @@ void tiled_descs(const TiledPlan& p, char* arena, std::vector<JobDesc>& d)
         d[j].beta = t.a.beta;
         d[j].fresh = p.fresh ? 1 : 0;
         d[j].shared = p.shared ? 1 : 0;
+        d[j].det = p.det ? 1 : 0;
+        d[j].pcap = (int32_t)p.partial_cap;
+        d[j].rcap = (int32_t)p.reduce_cap;
+        d[j].partials = p.det ? (void*)(arena + p.off_partials) : nullptr;
+        d[j].reduce = p.det ? (ReduceItem*)(arena + p.off_reduce) : nullptr;
     }
 }
 
@@ -1052,7 +1149,26 @@ int tiled_execute(const TiledPlan& p, char* arena, hipStream_t s)
     L.lds = p.lds;
     L.stage_min = p.stage_min;
     L.wg_per_cu = p.wg_per_cu;
-    return dispatch_owner(p.key, L, s);
+    MG_TRY(dispatch_owner(p.key, L, s));
+    if (p.det) {  // the split tiles, summed in item order
+        const int nb = (int)std::min<int64_t>(p.reduce_cap, 2048);
+        const int planar = p.key.planar ? 1 : 0;
+        const bool acc_d = p.key.stage == kStageF64 || p.key.acc64;
+        if (acc_d && p.key.out_type == MIGRID_F64)
+            hipLaunchKernelGGL((k_reduce_tiles<double, double>), dim3(nb), dim3(256), 0, s, L.jobs,
+                               L.qs, planar);
+        else if (acc_d)
+            hipLaunchKernelGGL((k_reduce_tiles<double, float>), dim3(nb), dim3(256), 0, s, L.jobs,
+                               L.qs, planar);
+        else if (p.key.out_type == MIGRID_F64)
+            hipLaunchKernelGGL((k_reduce_tiles<float, double>), dim3(nb), dim3(256), 0, s, L.jobs,
+                               L.qs, planar);
+        else
+            hipLaunchKernelGGL((k_reduce_tiles<float, float>), dim3(nb), dim3(256), 0, s, L.jobs,
+                               L.qs, planar);
+        MG_HIP(hipGetLastError());
+    }
+    return MIGRID_OK;
 }
 
 int launch_tiled(migrid_ctx* ctx, const GridArgs& a, const migrid_opts& o, hipStream_t s)

@@ -26,17 +26,27 @@ struct alignas(16) RecF64 {
 };
 
 // One unit of work: a job, a tile of it and an inclusive sub-kernel range.
+// pslot >= 0: the tile is split and the item writes its partial tile to
+// that slot for the ordered reduction (gather / window), else -1.
 struct Item {
     int32_t tile;
     int16_t klo, khi;
     int32_t job;
-    int32_t pad;
+    int32_t pslot;
 };
 
 struct QueueState {
     int32_t nitems;
     int32_t head;
-    int32_t pad[14];
+    int32_t npartial;  // partial-tile slots taken
+    int32_t nreduce;   // split tiles listed for the ordered reduction
+    int32_t pad[12];
+};
+
+// A split tile of the ordered-reduction mode: its n partial tiles are slots
+// [pslot, pslot + n), summed in that order (n = 0: nothing to reduce).
+struct ReduceItem {
+    int32_t job, tile, pslot, n;
 };
 
 struct TileGeom {
@@ -64,6 +74,13 @@ struct JobDesc {
                      // empty and split tiles were zeroed by the work-item pass
     int32_t shared;  // another job of the launch writes the same grid: every
                      // tile is flushed with atomics
+    // ordered reduction (gather / window, grids not shared): a split tile's
+    // items write partial tiles, summed per pixel in item order afterwards,
+    // so the grid is the same bit for bit on every run and no atomics are used
+    int32_t det;
+    int32_t pcap, rcap;  // partial-tile slots and reduce entries allocated
+    void* partials;      // pcap tiles of T x T complex accumulators
+    ReduceItem* reduce;  // rcap entries
 };
 
 // ---- host-side plan ---------------------------------------------------------
@@ -149,6 +166,9 @@ struct TiledPlan {
     size_t lds = 0;
     int stage_min = 0;  // tuning: staging threshold (0: default)
     int wg_per_cu = 0;  // tuning: persistent workgroups per CU (0: occupancy)
+    bool det = false;   // ordered reduction of split tiles (gather / window)
+    int64_t partial_cap = 0, reduce_cap = 0;
+    size_t off_partials = 0, off_reduce = 0;
 };
 
 struct OwnerLaunch {
