@@ -40,8 +40,12 @@ typedef struct { float x, y; } migrid_c64;   /* layout of float2 */
 /* ---- options ------------------------------------------------------------ */
 /* Strategy.  SCATTER is the reference default (grid_gpu.cu:778-786): one
  * hardware atomic per tap.  GATHER and WINDOW are the reference's other two
- * (:734-776), rebuilt as per-tile owner kernels.  TILED is the bin-sorted
- * LDS-tiled gridder with atomics only where a tile is flushed. */
+ * (:734-776), rebuilt as per-tile owner kernels that use no atomics: a dense
+ * tile split over several work items is summed from their partial tiles in
+ * a fixed order, so the grid is the same bit for bit on every run (for one
+ * grid per job; jobs of a batch sharing a grid flush atomically).  TILED is
+ * the bin-sorted LDS-tiled gridder with atomics only where a split tile is
+ * flushed. */
 #define MIGRID_SCATTER 0
 #define MIGRID_GATHER 1
 #define MIGRID_WINDOW 2
