@@ -493,6 +493,8 @@ def main():
     ap.add_argument("--vpw", type=int, default=0, help="tiled: visibilities per work item")
     ap.add_argument("--stage-min", type=int, default=0, help="tiled: staging threshold")
     ap.add_argument("--wg-per-cu", type=int, default=0, help="tiled: workgroups per CU")
+    ap.add_argument("--tap-smax", type=int, default=0,
+                    help="tiled: largest support gridded one tap per lane (0: 16, -1: never)")
     ap.add_argument("--dist", default="clustered", choices=["clustered", "uniform"])
     ap.add_argument("--mode", default="grid", choices=["grid", "batch", "serve", "copy", "degrid", "image"],
                     help="grid: one 8192^2 job (default); batch: many jobs in one persistent "
@@ -554,7 +556,7 @@ def main():
         "fp64_tiled_computed": dict(strategy="tiled", gcf_mode=migrid.GCF_COMPAT),
     }
     tune = dict(tile=args.tile, vis_per_wave=args.vpw, stage_min=args.stage_min,
-                wg_per_cu=args.wg_per_cu)
+                wg_per_cu=args.wg_per_cu, tap_smax=args.tap_smax)
     results = {}
     ref_grid = None       # the plain FP64 scatter's grid (table GCF)
     ref_computed = None   # the plain FP64 scatter's grid (COMPUTE_GCF)

@@ -92,7 +92,9 @@ typedef struct migrid_opts {
                                for a tile is read from HBM instead of staged in
                                LDS (0 = auto) */
     int32_t wg_per_cu;      /* TILED tuning: persistent workgroups per CU (0 = auto) */
-    int32_t reserved[6];
+    int32_t tap_smax;       /* TILED: supports up to this (at most 16) are gridded
+                               one tap per lane into an LDS tile; 0 = 16, -1 = never */
+    int32_t reserved[5];
 } migrid_opts;
 
 /* Fill *o with the defaults (scatter, default layout, table GCF, fresh grid). */

@@ -705,6 +705,163 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
     }
 }
 
+// ---- small supports: one tap per lane --------------------------------------------
+// At S <= 16 a wave's 16 x 16 block is mostly outside a visibility's S x S
+// window (at S = 8 about one executed tap in eight is real), and the
+// per-sub-kernel bookkeeping of 16 waves outweighs the few visibilities each
+// sees.  Here lane l owns taps l, l + 64, ... of the window (row t / S,
+// column t % S) and holds their GCF values for the current sub-kernel (the
+// next sub-kernel's are loaded meanwhile); a wave takes one visibility at a
+// time and adds its products into the workgroup's T x T tile in LDS with
+// LDS atomics.  The tile holds doubles for every arithmetic form: on gfx950
+// ds_add_f64 is cheap where ds_add_f32 is not, and the sums are the more
+// accurate for it (planar re / im planes, row pitch 8 mod 32 words, so the
+// rows of an 8- or 16-wide window spread over the banks).  An item's bin
+// bounds for all its sub-kernels are read into LDS at once, and the waves
+// take its visibilities round-robin over its sub-kernels and bin rows.  The
+// tile is flushed as the owner kernel flushes it.  LDS atomics add in no
+// fixed order, so this kernel serves the atomic TILED strategy.
+constexpr int kTapMaxS = 16;  // at most 4 taps per lane
+constexpr int kTapWaves = 16;  // a power of two (round-robin by mask)
+constexpr int kTapRows = 5;   // bin rows per tile: (T + S - 2) / 16 + 1 <= 5 at T <= 64, S <= 16
+__host__ __device__ constexpr int tap_pitch(int T) { return T + 4; }  // doubles
+
+// two workgroups per CU: 8 waves per SIMD, at most 64 VGPRs
+template <typename G, typename R, typename TO, bool PLANAR>
+__global__ __launch_bounds__(64 * kTapWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_taps(const JobDesc* __restrict__ jobs,
+                                                         const Item* __restrict__ items,
+                                                         QueueState* __restrict__ qs)
+{
+    using P = typename Stage<G>::P;  // product type
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* tacc = (double*)smem_raw;
+    __shared__ int s_item;
+    __shared__ int s_bnd[kSubKernels * kTapRows * 2];  // per sub-kernel and bin row: record range
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(&qs->head, 1);
+        __syncthreads();
+        const int it_idx = s_item;
+        if (it_idx >= qs->nitems) break;
+        const Item it = items[it_idx];
+        const int tile = __builtin_amdgcn_readfirstlane(it.tile);
+        const int klo = __builtin_amdgcn_readfirstlane(it.klo);
+        const int khi = __builtin_amdgcn_readfirstlane(it.khi);
+        const int job = __builtin_amdgcn_readfirstlane(it.job);
+        const TileGeom g = jobs[job].g;
+        ConstPtr<R> rec = (ConstPtr<R>)(uintptr_t)jobs[job].rec;
+        GlobalPtr<int32_t> offs = (GlobalPtr<int32_t>)(uintptr_t)jobs[job].offs;
+        GlobalPtr<G> gtab = (GlobalPtr<G>)(uintptr_t)jobs[job].gcf;  // table in the staged type
+        TO* __restrict__ grid = (TO*)jobs[job].grid;
+        TO* __restrict__ grid_lo = (TO*)jobs[job].grid_lo;
+        const bool atomic = jobs[job].split[tile] != 0;
+        const bool fresh = jobs[job].fresh != 0;
+        const int S = g.S, T = g.T, SS = S * S;
+        const int pitch = tap_pitch(T);
+        const int plane = T * pitch;
+        const int tx = tile % g.ntx, ty = tile / g.ntx;
+        const int X0 = tx * T, Y0 = ty * T;
+        // the bins of every visibility that reaches the tile (q in [X0, X0 + T + S - 2])
+        const int bxl = X0 / g.H, bxh = min(g.nb - 1, (X0 + T + S - 2) / g.H);
+        const int byl = Y0 / g.H, byh = min(g.nb - 1, (Y0 + T + S - 2) / g.H);
+        const int nrow = byh - byl + 1;
+        for (int e = tid; e < 2 * plane; e += blockDim.x) tacc[e] = 0.0;
+        for (int e = tid; e < (khi - klo + 1) * nrow * 2; e += blockDim.x) {
+            const int kk = klo + e / (nrow * 2), t = (e >> 1) % nrow;
+            const int64_t key = ((int64_t)kk * g.nb + byl + t) * g.nb;
+            s_bnd[(kk * kTapRows + t) * 2 + (e & 1)] = offs[(e & 1) ? key + bxh + 1 : key + bxl];
+        }
+        // per tap: row, column and LDS offset relative to the window's corner
+        const int ntap = (SS + 63) / 64;
+        int tr[4], tc[4], to[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            tr[i] = (lane + 64 * i) / S;
+            tc[i] = (lane + 64 * i) % S;
+            to[i] = tr[i] * pitch + tc[i];
+        }
+        uint64_t nz_mask;
+        {
+            const int cnt =
+                (lane >= klo && lane <= khi) ? jobs[job].tcount[(int64_t)tile * kSubKernels + lane] : 0;
+            nz_mask = __ballot(cnt > 0);
+        }
+        auto load_taps = [&](int kk, P(&gr)[4], P(&gi)[4]) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                gr[i] = gi[i] = (P)0;
+                if (i < ntap && lane + 64 * i < SS)
+                    Stage<G>::get(gload<G>(gtab, (int64_t)kk * SS + lane + 64 * i), gr[i], gi[i]);
+            }
+        };
+        P gr[4], gi[4];
+        if (nz_mask) load_taps(__builtin_ctzll(nz_mask), gr, gi);
+        __syncthreads();  // tile cleared, bounds listed
+        int handed = 0;   // visibilities of the item handed out so far, mod kTapWaves
+        for (uint64_t todo = nz_mask; todo; todo &= todo - 1) {
+            const int k = __builtin_ctzll(todo);
+            P ngr[4], ngi[4];
+            {
+                const uint64_t rest = todo & (todo - 1);
+                if (rest) load_taps(__builtin_ctzll(rest), ngr, ngi);
+            }
+            // lanes 2t and 2t + 1 hold the record range of bin row t
+            const int bv = lane < 2 * nrow ? s_bnd[k * kTapRows * 2 + lane] : 0;
+#pragma unroll
+            for (int t = 0; t < kTapRows; t++) {
+                if (t >= nrow) break;
+                const int r0 = __builtin_amdgcn_readlane(bv, 2 * t);
+                const int r1 = __builtin_amdgcn_readlane(bv, 2 * t + 1);
+                // this wave's visibilities: running index = w (mod kTapWaves)
+                const int i0 = (w - handed) & (kTapWaves - 1);
+                for (int r = r0 + i0; r < r1; r += kTapWaves) {
+                    const uint32_t q = rec[r].q;
+                    const P vr = (P)rec[r].vr, vi = (P)rec[r].vi;
+                    // tap (row, column) lands on pixel q - (S - 1) + (column, row)
+                    const int sx = (int)(q & 0xffffu) - (S - 1) - X0;
+                    const int sy = (int)(q >> 16) - (S - 1) - Y0;
+                    const int se = sy * pitch + sx;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        if (i >= ntap) break;
+                        const int x = sx + tc[i], y = sy + tr[i];
+                        if (lane + 64 * i < SS && (unsigned)x < (unsigned)T &&
+                            (unsigned)y < (unsigned)T) {
+                            const int e = se + to[i];
+                            atomicAdd(&tacc[e], (double)(vr * gr[i] - vi * gi[i]));
+                            atomicAdd(&tacc[plane + e], (double)(vr * gi[i] + vi * gr[i]));
+                        }
+                    }
+                }
+                handed = (handed + (r1 - r0)) & (kTapWaves - 1);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                gr[i] = ngr[i];
+                gi[i] = ngi[i];
+            }
+        }
+        __syncthreads();
+        const int64_t NN = (int64_t)g.N * g.N;
+        for (int e = tid; e < T * T; e += blockDim.x) {
+            const int ey = e / T, ex = e % T;
+            const int x = X0 + ex, y = Y0 + ey;
+            if (x >= g.N || y >= g.N) continue;
+            const int le = ey * pitch + ex;
+            const int64_t p = (int64_t)y * g.N + x;
+            if (PLANAR)
+                flush_px(grid, grid_lo, p, NN + p, tacc[le], tacc[plane + le], atomic, fresh);
+            else
+                flush_px(grid, grid_lo, 2 * p, 2 * p + 1, tacc[le], tacc[plane + le], atomic, fresh);
+        }
+        __syncthreads();  // the tile, the bounds and s_item are reused
+    }
+}
+
 // Ordered reduction of split tiles (gather / window): each listed tile's
 // partial tiles are summed per pixel in item order and flushed once -- a
 // plain store into a fresh grid, else read-add-write (the tile is this
@@ -835,6 +992,43 @@ int run_modes(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
     return run_shape<G, TA, R, TI, TO, false, 2>(k.T, L, s);
 }
 
+template <typename G, typename R, typename TO, bool PLANAR>
+int run_taps(const OwnerLaunch& L, hipStream_t s)
+{
+    auto kern = k_taps<G, R, TO, PLANAR>;
+    if (L.lds > kMaxLds)
+        return fail(MIGRID_ENOTSUP, "tiled: the LDS tile needs %zu bytes (at most %zu)", L.lds,
+                    kMaxLds);
+    MG_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)L.lds));
+    int per_cu = 0;
+    MG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * kTapWaves, L.lds));
+    if (per_cu < 1) return fail(MIGRID_ENOTSUP, "tiled: tap kernel does not fit (lds=%zu)", L.lds);
+    int dev = 0, ncu = 0;
+    MG_HIP(hipGetDevice(&dev));
+    MG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (getenv("MIGRID_VERBOSE")) {
+        hipFuncAttributes fa;
+        MG_HIP(hipFuncGetAttributes(&fa, (const void*)kern));
+        fprintf(stderr,
+                "[migrid] tap launch: %d CUs x %d WG/CU (occupancy %d), lds dyn %zu, vgpr %d, "
+                "spill %zu\n",
+                ncu, L.wg_per_cu > 0 ? L.wg_per_cu : per_cu, per_cu, L.lds, fa.numRegs,
+                (size_t)fa.localSizeBytes);
+    }
+    if (L.wg_per_cu > 0) per_cu = std::min(per_cu, L.wg_per_cu);
+    hipLaunchKernelGGL(kern, dim3(ncu * per_cu), dim3(64 * kTapWaves), L.lds, s, L.jobs, L.items,
+                       L.qs);
+    MG_HIP(hipGetLastError());
+    return MIGRID_OK;
+}
+
+template <typename G, typename R, typename TO>
+int run_taps_layout(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
+{
+    return k.planar ? run_taps<G, R, TO, true>(L, s) : run_taps<G, R, TO, false>(L, s);
+}
+
 template <typename TI, typename R>
 void launch_pack(const PackJob* table, int n, int64_t max_npts, int vt, const int32_t* perm, R* rec,
                  hipStream_t s)
@@ -877,6 +1071,22 @@ int dispatch_owner(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
                     : run_modes<float2, float, RecF32, float, float>(k, L, s);
     default:
         return run_modes<double2, double, RecF64, double, double>(k, L, s);
+    }
+}
+
+int dispatch_taps(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
+{
+    if (k.gm != 0) return fail(MIGRID_ENOTSUP, "tap kernel: table GCF only");
+    const bool out64 = k.out_type == MIGRID_F64;
+    switch (k.stage) {
+    case kStageBf16:
+        return out64 ? run_taps_layout<Bf16x2, RecF32, double>(k, L, s)
+                     : run_taps_layout<Bf16x2, RecF32, float>(k, L, s);
+    case kStageF32:  // float products (summed in double in the tile, as acc64 asks)
+        return out64 ? run_taps_layout<float2, RecF32, double>(k, L, s)
+                     : run_taps_layout<float2, RecF32, float>(k, L, s);
+    default:
+        return run_taps_layout<double2, RecF64, double>(k, L, s);
     }
 }
 
@@ -1025,6 +1235,11 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.wg_per_cu = o.wg_per_cu;
     const size_t gsz = p.key.stage == kStageBf16 ? 4 : p.key.stage == kStageF32 ? 8 : 16;
     p.lds = (size_t)stage_rows(p.smax) * stage_pitch(p.smax) * gsz;
+    // small supports: one tap per lane into an LDS tile (atomic strategy, table GCF)
+    const int tap_smax = o.tap_smax == 0 ? kTapMaxS : std::min(o.tap_smax, kTapMaxS);
+    p.taps = o.strategy == MIGRID_TILED && p.key.gm == 0 && p.smax <= tap_smax &&
+             (p.key.T + p.smax - 2) / 16 + 1 <= kTapRows;
+    if (p.taps) p.lds = (size_t)2 * p.key.T * tap_pitch(p.key.T) * sizeof(double);
     return MIGRID_OK;
 }
 
@@ -1149,7 +1364,10 @@ int tiled_execute(const TiledPlan& p, char* arena, hipStream_t s)
     L.lds = p.lds;
     L.stage_min = p.stage_min;
     L.wg_per_cu = p.wg_per_cu;
-    MG_TRY(dispatch_owner(p.key, L, s));
+    if (p.taps)
+        MG_TRY(dispatch_taps(p.key, L, s));
+    else
+        MG_TRY(dispatch_owner(p.key, L, s));
     if (p.det) {  // the split tiles, summed in item order
         const int nb = (int)std::min<int64_t>(p.reduce_cap, 2048);
         const int planar = p.key.planar ? 1 : 0;

@@ -169,6 +169,7 @@ struct TiledPlan {
     bool det = false;   // ordered reduction of split tiles (gather / window)
     int64_t partial_cap = 0, reduce_cap = 0;
     size_t off_partials = 0, off_reduce = 0;
+    bool taps = false;  // small supports: the one-tap-per-lane kernel (k_taps)
 };
 
 struct OwnerLaunch {
@@ -193,5 +194,7 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
 // The persistent owner kernel over all queued items.
 int tiled_execute(const TiledPlan& p, char* arena, hipStream_t s);
 int dispatch_owner(const KernelKey& k, const OwnerLaunch& L, hipStream_t s);
+// The small-support kernel (one tap per lane, LDS tile) over all queued items.
+int dispatch_taps(const KernelKey& k, const OwnerLaunch& L, hipStream_t s);
 
 }  // namespace migrid

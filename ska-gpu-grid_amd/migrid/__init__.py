@@ -52,7 +52,8 @@ class Opts(ctypes.Structure):
         ("gcf_beta", ctypes.c_double),
         ("stage_min", ctypes.c_int32),
         ("wg_per_cu", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("tap_smax", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -184,11 +185,12 @@ def version():
 
 def make_opts(strategy="scatter", layout=LAYOUT_DEFAULT, gcf_mode=GCF_TABLE, arith=ARITH_NATIVE,
               accumulate=False, debug_coverage=False, tile=0, vis_per_wave=0, sigma=0.0,
-              beta=0.05, stage_min=0, wg_per_cu=0):
+              beta=0.05, stage_min=0, wg_per_cu=0, tap_smax=0):
     o = Opts()
     lib().migrid_opts_default(ctypes.byref(o))
     o.stage_min = stage_min
     o.wg_per_cu = wg_per_cu
+    o.tap_smax = tap_smax
     o.strategy = STRATEGIES[strategy] if isinstance(strategy, str) else int(strategy)
     o.layout = layout
     o.gcf_mode = gcf_mode

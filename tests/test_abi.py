@@ -71,9 +71,9 @@ def test_bins_per_axis():
 
 def test_make_opts_keeps_every_field():
     o = migrid.make_opts(strategy="tiled", tile=32, vis_per_wave=777, stage_min=5, wg_per_cu=2,
-                         accumulate=True, sigma=3.0, beta=0.1)
-    assert (o.strategy, o.tile, o.vis_per_wave, o.stage_min, o.wg_per_cu, o.accumulate) == \
-        (migrid.TILED, 32, 777, 5, 2, 1)
+                         accumulate=True, sigma=3.0, beta=0.1, tap_smax=-1)
+    assert (o.strategy, o.tile, o.vis_per_wave, o.stage_min, o.wg_per_cu, o.accumulate,
+            o.tap_smax) == (migrid.TILED, 32, 777, 5, 2, 1, -1)
     assert o.gcf_sigma == 3.0 and o.gcf_beta == 0.1
 
 

@@ -1,0 +1,195 @@
+"""GPU parity of the small-support kernel: the TILED strategy at S <= 16 grids
+one tap per lane into an LDS tile (k_taps) instead of 16 x 16 wave blocks.
+
+Against the CPU oracle: FP64 within 1e-12 relative max-norm; FP32 tiles
+within 1e-5; BF16 GCF tiles within 1e-5 of the oracle on the bf16-rounded
+table; float pairs within 2e-5.  Against the owner kernel on the same inputs
+(tap_smax = -1 turns the tap kernel off): FP64 within 1e-13.  Dense clusters
+with tiny work items force split tiles (atomic flushes onto zeroed tiles);
+a batch of small supports replays as a graph.  MIGRID_VERBOSE names the
+kernel each launch used, which the tests read to be sure which path ran.
+"""
+import numpy as np
+import pytest
+
+import migrid
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def torch():
+    import torch as t
+    return t
+
+
+def dev(a):
+    return torch().from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def cplx(p):
+    return p[0] + 1j * p[1]
+
+
+def bf16_round(a):
+    f = np.ascontiguousarray(a, np.float32)
+    b = f.view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+    return b.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def inputs(N, S, n, dense, seed):
+    rng = np.random.default_rng(seed)
+    if dense:
+        uv = rng.uniform(0, N, size=2) + rng.normal(scale=2.0, size=(n, 2))
+    else:
+        uv = rng.uniform(-S, N + S, size=(n, 2))
+    return uv.astype(np.float32).astype(np.float64), rng.normal(size=(n, 2))
+
+
+def grid_on(uv, vals, gcf, N, S, out="f64", in_dtype="f64", planar=False, base=None, **kw):
+    t = torch()
+    dt = t.float64 if out == "f64" else t.float32
+    shape = (2, N, N) if planar else (N, N, 2)
+    grid = dev(base.astype(np.float64)).to(dt) if base is not None else t.empty(shape, dtype=dt,
+                                                                               device="cuda")
+    idt = np.float64 if in_dtype == "f64" else np.float32
+    ctx = migrid.Context()
+    ctx.grid(grid, dev(uv.astype(idt)), dev(vals.astype(idt)), dev(gcf.astype(idt)), N, S,
+             layout=migrid.LAYOUT_PLANAR if planar else migrid.LAYOUT_INTERLEAVED,
+             accumulate=base is not None, **kw)
+    t.cuda.synchronize()
+    g = grid.cpu().numpy().astype(np.float64)
+    return g[0] + 1j * g[1] if planar else g[..., 0] + 1j * g[..., 1]
+
+
+def rel(got, want):
+    m = np.abs(want).max(initial=0.0)
+    if m == 0.0:
+        return float(np.abs(got).max(initial=0.0))
+    return float(np.abs(got - want).max() / m)
+
+
+CASES = [(40, 2, 500, False, 16), (100, 4, 3000, True, 32), (129, 8, 4000, False, 64),
+         (64, 8, 6000, True, 64), (90, 10, 2000, False, 32), (257, 12, 5000, True, 64),
+         (200, 16, 5000, False, 64), (31, 16, 300, False, 16), (5, 6, 50, False, 16)]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "N%d_S%d_n%d%s_T%d" % (
+    c[0], c[1], c[2], "_dense" if c[3] else "", c[4]))
+def test_taps_against_oracle(case, capfd, monkeypatch):
+    N, S, n, dense, T = case
+    print(f"[taps] case {case}", flush=True)
+    monkeypatch.setenv("MIGRID_VERBOSE", "1")
+    uv, vals = inputs(N, S, n, dense, seed=N * 100 + S)
+    gcf = O.config_gcf(S)
+    want = cplx(O.grid_zz(uv, vals, N, gcf, S))
+    g = grid_on(uv, vals, gcf, N, S, strategy="tiled", tile=T)
+    assert rel(g, want) < 1e-12, "fp64"
+    assert "tap launch" in capfd.readouterr().err
+    # FP32 tiles, planar, accumulating into a filled grid
+    base = np.random.default_rng(S).normal(size=(2, N, N))
+    g = grid_on(uv, vals, gcf, N, S, planar=True, base=base, strategy="tiled", tile=T,
+                arith=migrid.ARITH_F32) - cplx(base)
+    m = max(np.abs(want).max(initial=0.0), 1e-300)
+    assert np.abs(g - want).max(initial=0.0) <= 1e-5 * m + 1e-12 * np.abs(base).max()
+    # BF16 GCF tiles against the oracle on the rounded table
+    wantb = cplx(O.grid_zz(uv, vals, N, bf16_round(gcf), S))
+    g = grid_on(uv, vals, gcf, N, S, strategy="tiled", tile=T, arith=migrid.ARITH_F32_BF16GCF)
+    assert rel(g, wantb) < 1e-5, "bf16"
+    # float pairs: <double2,float2> (float products summed in double), <float2,float2>
+    g = grid_on(uv, vals, gcf, N, S, in_dtype="f32", strategy="tiled", tile=T)
+    assert rel(g, want) < 2e-5, "zc"
+    g = grid_on(uv, vals, gcf, N, S, out="f32", in_dtype="f32", strategy="tiled", tile=T)
+    assert rel(g, want) < 2e-5, "cc"
+    assert capfd.readouterr().err.count("tap launch") == 4
+
+
+@pytest.mark.parametrize("S", [2, 8, 12, 16])
+def test_taps_match_owner_kernel(S, capfd, monkeypatch):
+    N, n = 160, 6000
+    print(f"[taps vs owner] S={S}", flush=True)
+    monkeypatch.setenv("MIGRID_VERBOSE", "1")
+    uv, vals = inputs(N, S, n, True, seed=S)
+    gcf = O.config_gcf(S)
+    a = grid_on(uv, vals, gcf, N, S, strategy="tiled")
+    b = grid_on(uv, vals, gcf, N, S, strategy="tiled", tap_smax=-1)
+    err = capfd.readouterr().err
+    assert err.count("tap launch") == 1 and err.count("owner launch") == 1
+    assert rel(a, b) < 1e-13
+
+
+def test_tap_smax_threshold(capfd, monkeypatch):
+    """tap_smax caps the supports sent to the tap kernel; S = 18 never uses it."""
+    monkeypatch.setenv("MIGRID_VERBOSE", "1")
+    N = 120
+    for S, opt, path in ((12, 8, "owner"), (12, 12, "tap"), (8, 0, "tap"), (18, 0, "owner"),
+                         (18, 64, "owner")):
+        print(f"[tap_smax] S={S} tap_smax={opt}", flush=True)
+        uv, vals = inputs(N, S, 2000, False, seed=S + opt)
+        gcf = O.config_gcf(S)
+        g = grid_on(uv, vals, gcf, N, S, strategy="tiled", tap_smax=opt)
+        assert rel(g, cplx(O.grid_zz(uv, vals, N, gcf, S))) < 1e-12
+        err = capfd.readouterr().err
+        assert f"{path} launch" in err, (S, opt, err[-300:])
+
+
+def test_gather_and_window_keep_the_owner_kernel(capfd, monkeypatch):
+    monkeypatch.setenv("MIGRID_VERBOSE", "1")
+    N, S = 100, 8
+    uv, vals = inputs(N, S, 3000, True, seed=1)
+    gcf = O.config_gcf(S)
+    want = cplx(O.grid_zz(uv, vals, N, gcf, S))
+    for strategy in ("gather", "window"):
+        planar = strategy == "window"
+        a = grid_on(uv, vals, gcf, N, S, planar=planar, strategy=strategy)
+        b = grid_on(uv, vals, gcf, N, S, planar=planar, strategy=strategy)
+        assert rel(a, want) < 1e-12 and np.array_equal(a, b)
+    err = capfd.readouterr().err
+    assert "tap launch" not in err and "owner launch" in err
+
+
+@pytest.mark.parametrize("vpw", [16, 64])
+def test_split_tiles(vpw):
+    """Tiny work items split the dense tiles over many items, which flush with
+    atomics onto tiles the work-item pass zeroed."""
+    N, S = 96, 8
+    print(f"[taps split] vis_per_wave={vpw}", flush=True)
+    uv, vals = inputs(N, S, 20000, True, seed=vpw)
+    gcf = O.config_gcf(S)
+    want = cplx(O.grid_zz(uv, vals, N, gcf, S))
+    for arith, tol in ((migrid.ARITH_NATIVE, 1e-12), (migrid.ARITH_F32, 1e-5)):
+        g = grid_on(uv, vals, gcf, N, S, strategy="tiled", vis_per_wave=vpw, arith=arith)
+        assert rel(g, want) < tol, arith
+
+
+def test_small_support_batch_and_graph(capfd, monkeypatch):
+    t = torch()
+    monkeypatch.setenv("MIGRID_VERBOSE", "1")
+    shapes = [(128, 8, 3000), (200, 16, 5000), (64, 2, 800), (150, 12, 0), (300, 4, 4000)]
+    jobs, wants = [], []
+    for i, (N, S, n) in enumerate(shapes):
+        uv, vals = inputs(N, S, n, i % 2 == 1, seed=40 + i)
+        gcf = O.config_gcf(S)
+        wants.append(cplx(O.grid_zz(uv, vals, N, gcf, S)))
+        jobs.append(dict(grid=t.empty((N, N, 2), dtype=t.float64, device="cuda"), uv=dev(uv),
+                         vals=dev(vals), gcf=dev(gcf), N=N, S=S))
+    ctx = migrid.Context()
+    for arith, tol in ((migrid.ARITH_NATIVE, 1e-12), (migrid.ARITH_F32, 1e-5)):
+        print(f"[taps batch] arith {arith}", flush=True)
+        b = migrid.Batch(ctx, jobs, arith=arith)
+        stream = t.cuda.Stream()
+        for mode in ("run", "graph", "graph", "graph"):
+            for j in jobs:
+                j["grid"].fill_(7.0)
+            t.cuda.synchronize()
+            if mode == "run":
+                b.run(stream)
+            else:
+                b.graph_launch(stream)
+            t.cuda.synchronize()
+            for j, w in zip(jobs, wants):
+                g = j["grid"].cpu().numpy()
+                assert rel(g[..., 0] + 1j * g[..., 1], w) < tol, (mode, j["N"], j["S"])
+        b.close()
+    assert "tap launch" in capfd.readouterr().err
