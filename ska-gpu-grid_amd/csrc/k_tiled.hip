@@ -717,12 +717,14 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
 // ds_add_f64 is cheap where ds_add_f32 is not, and the sums are the more
 // accurate for it (planar re / im planes, row pitch 8 mod 32 words, so the
 // rows of an 8- or 16-wide window spread over the banks).  An item's bin
-// bounds for all its sub-kernels are read into LDS at once, and the waves
-// take its visibilities round-robin over its sub-kernels and bin rows.  The
+// bounds for all its sub-kernels are read into LDS at once.  A sub-kernel
+// with many visibilities in the tile is shared by the waves round-robin; the
+// sparse ones are dealt out whole, one wave each.  The
 // tile is flushed as the owner kernel flushes it.  LDS atomics add in no
 // fixed order, so this kernel serves the atomic TILED strategy.
 constexpr int kTapMaxS = 16;  // at most 4 taps per lane
 constexpr int kTapWaves = 16;  // a power of two (round-robin by mask)
+constexpr int kTapShare = 32;  // visibilities of a sub-kernel in a tile from which all waves share it
 constexpr int kTapRows = 5;   // bin rows per tile: (T + S - 2) / 16 + 1 <= 5 at T <= 64, S <= 16
 __host__ __device__ constexpr int tap_pitch(int T) { return T + 4; }  // doubles
 
@@ -784,11 +786,18 @@ __global__ __launch_bounds__(64 * kTapWaves) __attribute__((amdgpu_waves_per_eu(
             tc[i] = (lane + 64 * i) % S;
             to[i] = tr[i] * pitch + tc[i];
         }
-        uint64_t nz_mask;
+        // Sub-kernels with many visibilities in the tile are shared by all
+        // waves (round-robin by visibility); the others are dealt out whole,
+        // one wave each, so their bookkeeping is not repeated by every wave.
+        uint64_t nz_mask, big_mask;
         {
             const int cnt =
                 (lane >= klo && lane <= khi) ? jobs[job].tcount[(int64_t)tile * kSubKernels + lane] : 0;
-            nz_mask = __ballot(cnt > 0);
+            const uint64_t small_all = __ballot(cnt > 0 && cnt < kTapShare);
+            big_mask = __ballot(cnt >= kTapShare);
+            const int rank = __builtin_popcountll(small_all & ((1ull << lane) - 1));
+            const uint64_t mine = __ballot(((small_all >> lane) & 1) && (rank & (kTapWaves - 1)) == w);
+            nz_mask = big_mask | mine;
         }
         auto load_taps = [&](int kk, P(&gr)[4], P(&gi)[4]) {
 #pragma unroll
@@ -809,6 +818,7 @@ __global__ __launch_bounds__(64 * kTapWaves) __attribute__((amdgpu_waves_per_eu(
                 const uint64_t rest = todo & (todo - 1);
                 if (rest) load_taps(__builtin_ctzll(rest), ngr, ngi);
             }
+            const bool shared = (big_mask >> k) & 1;
             // lanes 2t and 2t + 1 hold the record range of bin row t
             const int bv = lane < 2 * nrow ? s_bnd[k * kTapRows * 2 + lane] : 0;
 #pragma unroll
@@ -816,9 +826,11 @@ __global__ __launch_bounds__(64 * kTapWaves) __attribute__((amdgpu_waves_per_eu(
                 if (t >= nrow) break;
                 const int r0 = __builtin_amdgcn_readlane(bv, 2 * t);
                 const int r1 = __builtin_amdgcn_readlane(bv, 2 * t + 1);
-                // this wave's visibilities: running index = w (mod kTapWaves)
-                const int i0 = (w - handed) & (kTapWaves - 1);
-                for (int r = r0 + i0; r < r1; r += kTapWaves) {
+                // a shared sub-kernel: this wave's visibilities have running
+                // index = w (mod kTapWaves); a dealt one: all of them
+                const int i0 = shared ? (w - handed) & (kTapWaves - 1) : 0;
+                const int step = shared ? kTapWaves : 1;
+                for (int r = r0 + i0; r < r1; r += step) {
                     const uint32_t q = rec[r].q;
                     const P vr = (P)rec[r].vr, vi = (P)rec[r].vi;
                     // tap (row, column) lands on pixel q - (S - 1) + (column, row)
@@ -837,7 +849,7 @@ __global__ __launch_bounds__(64 * kTapWaves) __attribute__((amdgpu_waves_per_eu(
                         }
                     }
                 }
-                handed = (handed + (r1 - r0)) & (kTapWaves - 1);
+                if (shared) handed = (handed + (r1 - r0)) & (kTapWaves - 1);
             }
 #pragma unroll
             for (int i = 0; i < 4; i++) {
