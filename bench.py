@@ -198,7 +198,7 @@ def main_batch(args, torch, migrid):
     print(f"# batch: {len(jobs)} jobs, {total} visibilities", file=sys.stderr, flush=True)
     ctx = migrid.Context()
     t0 = time.perf_counter()
-    best, best_ms = migrid.autotune(ctx, jobs, arith=migrid.ARITH_F32)
+    best, best_ms = migrid.autotune(ctx, jobs, arith=migrid.ARITH_F32, tap_smax=args.tap_smax)
     tune_s = time.perf_counter() - t0
     print(f"# autotune: tile {best.tile} vis/item {best.vis_per_wave} wg/CU {best.wg_per_cu} "
           f"stage_min {best.stage_min} {best_ms:.2f} ms ({tune_s:.1f} s)", file=sys.stderr, flush=True)

@@ -224,15 +224,21 @@ __device__ void zero_tile(TO* grid, int N, int X0, int Y0, int T, bool planar)
 // ---- work items -----------------------------------------------------------------
 // One wave per tile; lane = sub-kernel k.  Cuts the tile's sub-kernels into
 // items of about `target` visibilities (target <= 0: one item per tile).
-// Every job in one launch: blockIdx.y = job.  For a fresh grid the wave
+// Every job in one launch: blockIdx.y = job; a small-support job's items go
+// to the small-support kernel's queue.  For a fresh grid the wave
 // zeroes its tile when the owner kernel will not store it whole: no items
 // (nothing stores it) or several (they add to it atomically).
 template <typename TO>
 __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs, int64_t target,
-                                               int planar, Item* __restrict__ items,
-                                               QueueState* __restrict__ qs)
+                                               int planar, Item* __restrict__ items_o,
+                                               QueueState* __restrict__ qs_o,
+                                               Item* __restrict__ items_t,
+                                               QueueState* __restrict__ qs_t)
 {
     const int job = blockIdx.y;
+    // the owner kernel's queue, or the small-support kernel's
+    Item* __restrict__ items = jobs[job].taps ? items_t : items_o;
+    QueueState* __restrict__ qs = jobs[job].taps ? qs_t : qs_o;
     const TileGeom g = jobs[job].g;
     const int32_t* __restrict__ tcount = jobs[job].tcount;
     uint8_t* __restrict__ split = (uint8_t*)jobs[job].split;
@@ -1122,7 +1128,10 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p = TiledPlan();
     p.jobs.resize(n);
     size_t off = 0;
-    int64_t item_cap = 0;
+    int64_t item_cap = 0, item_cap_t = 0;
+    int smax_owner = 0;
+    // small supports: one tap per lane into an LDS tile (atomic strategy, table GCF)
+    const int tap_smax = o.tap_smax == 0 ? kTapMaxS : std::min(o.tap_smax, kTapMaxS);
     for (int j = 0; j < n; j++) {
         const GridArgs& x = a[j];
         if (x.S > 64) return fail(MIGRID_ENOTSUP, "tiled strategies support S <= 64 (S=%d)", x.S);
@@ -1187,9 +1196,16 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
             p.nconv++;
             p.max_gcf = std::max<int64_t>(p.max_gcf, (int64_t)kSubKernels * x.S * x.S);
         }
-        item_cap += ntiles * kSubKernels;
+        t.taps = o.strategy == MIGRID_TILED && k.gm == 0 && x.S <= tap_smax &&
+                 (tc.T + x.S - 2) / 16 + 1 <= kTapRows;
+        (t.taps ? item_cap_t : item_cap) += ntiles * kSubKernels;
+        if (t.taps)
+            p.taps = true;
+        else
+            smax_owner = std::max(smax_owner, x.S);
         p.smax = std::max(p.smax, x.S);
     }
+    p.owner = smax_owner > 0;
     if (p.K >= ((int64_t)1 << 31) || p.E >= ((int64_t)1 << 31))
         return fail(MIGRID_ENOTSUP, "tiled: %lld keys / %lld visibilities in one plan (limit 2^31)",
                     (long long)p.K, (long long)p.E);
@@ -1209,9 +1225,12 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.off_desc = off;
     off += align256(sizeof(JobDesc) * n);
     p.off_items = off;
-    off += align256((size_t)item_cap * sizeof(Item));
-    p.off_qs = off;
-    off += align256(sizeof(QueueState));
+    off += align256((size_t)(item_cap > 0 ? item_cap : 1) * sizeof(Item));
+    p.off_items_t = off;
+    off += align256((size_t)(item_cap_t > 0 ? item_cap_t : 1) * sizeof(Item));
+    p.off_qs = off;  // the owner kernel's queue, then the small-support kernel's
+    p.off_qs_t = off + sizeof(QueueState);
+    off += align256(2 * sizeof(QueueState));
     // jobs that write one grid flush atomically, into a grid cleared beforehand
     {
         std::vector<const void*> grids;
@@ -1243,15 +1262,12 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.arena_bytes = off;
     p.bin_bytes = bin_scratch_bytes(p.E, p.K);
     p.item_cap = item_cap;
+    p.item_cap_t = item_cap_t;
     p.stage_min = o.stage_min;
     p.wg_per_cu = o.wg_per_cu;
     const size_t gsz = p.key.stage == kStageBf16 ? 4 : p.key.stage == kStageF32 ? 8 : 16;
-    p.lds = (size_t)stage_rows(p.smax) * stage_pitch(p.smax) * gsz;
-    // small supports: one tap per lane into an LDS tile (atomic strategy, table GCF)
-    const int tap_smax = o.tap_smax == 0 ? kTapMaxS : std::min(o.tap_smax, kTapMaxS);
-    p.taps = o.strategy == MIGRID_TILED && p.key.gm == 0 && p.smax <= tap_smax &&
-             (p.key.T + p.smax - 2) / 16 + 1 <= kTapRows;
-    if (p.taps) p.lds = (size_t)2 * p.key.T * tap_pitch(p.key.T) * sizeof(double);
+    p.lds = p.owner ? (size_t)stage_rows(smax_owner) * stage_pitch(smax_owner) * gsz : 0;
+    p.lds_taps = p.taps ? (size_t)2 * p.key.T * tap_pitch(p.key.T) * sizeof(double) : 0;
     return MIGRID_OK;
 }
 
@@ -1273,6 +1289,7 @@ void tiled_descs(const TiledPlan& p, char* arena, std::vector<JobDesc>& d)
         d[j].fresh = p.fresh ? 1 : 0;
         d[j].shared = p.shared ? 1 : 0;
         d[j].det = p.det ? 1 : 0;
+        d[j].taps = t.taps ? 1 : 0;
         d[j].pcap = (int32_t)p.partial_cap;
         d[j].rcap = (int32_t)p.reduce_cap;
         d[j].partials = p.det ? (void*)(arena + p.off_partials) : nullptr;
@@ -1293,7 +1310,7 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
     Item* items = (Item*)(arena + p.off_items);
     int32_t* perm = (int32_t*)(arena + p.off_perm);
     int32_t* offs = (int32_t*)(arena + p.off_offs);
-    MG_TRY(fill_u32(qs, 0u, sizeof(QueueState) / 4, s));
+    MG_TRY(fill_u32(qs, 0u, 2 * sizeof(QueueState) / 4, s));  // both queues
     // job tables
     std::vector<BinJob> bj(n);
     std::vector<PackJob> pj(n);
@@ -1356,12 +1373,14 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
     const int cb = (int)std::min<int64_t>(max_ntx * kSubKernels, n < 8 ? 8192 : 65536 / n);
     hipLaunchKernelGGL(k_tile_counts, dim3(cb, n), dim3(128), 0, s, ddesc);
     const dim3 ig((unsigned)((max_tiles + 3) / 4), n);
+    Item* items_t = (Item*)(arena + p.off_items_t);
+    QueueState* qs_t = (QueueState*)(arena + p.off_qs_t);
     if (p.key.out_type == MIGRID_F64)
         hipLaunchKernelGGL(k_items<double>, ig, dim3(256), 0, s, ddesc, p.target,
-                           p.key.planar ? 1 : 0, items, qs);
+                           p.key.planar ? 1 : 0, items, qs, items_t, qs_t);
     else
         hipLaunchKernelGGL(k_items<float>, ig, dim3(256), 0, s, ddesc, p.target,
-                           p.key.planar ? 1 : 0, items, qs);
+                           p.key.planar ? 1 : 0, items, qs, items_t, qs_t);
     MG_HIP(hipGetLastError());
     return MIGRID_OK;
 }
@@ -1376,10 +1395,14 @@ int tiled_execute(const TiledPlan& p, char* arena, hipStream_t s)
     L.lds = p.lds;
     L.stage_min = p.stage_min;
     L.wg_per_cu = p.wg_per_cu;
-    if (p.taps)
-        MG_TRY(dispatch_taps(p.key, L, s));
-    else
-        MG_TRY(dispatch_owner(p.key, L, s));
+    if (p.taps) {
+        OwnerLaunch Lt = L;
+        Lt.items = (const Item*)(arena + p.off_items_t);
+        Lt.qs = (QueueState*)(arena + p.off_qs_t);
+        Lt.lds = p.lds_taps;
+        MG_TRY(dispatch_taps(p.key, Lt, s));
+    }
+    if (p.owner) MG_TRY(dispatch_owner(p.key, L, s));
     if (p.det) {  // the split tiles, summed in item order
         const int nb = (int)std::min<int64_t>(p.reduce_cap, 2048);
         const int planar = p.key.planar ? 1 : 0;

@@ -78,6 +78,7 @@ struct JobDesc {
     // items write partial tiles, summed per pixel in item order afterwards,
     // so the grid is the same bit for bit on every run and no atomics are used
     int32_t det;
+    int32_t taps;        // gridded by the small-support kernel (its own item queue)
     int32_t pcap, rcap;  // partial-tile slots and reduce entries allocated
     void* partials;      // pcap tiles of T x T complex accumulators
     ReduceItem* reduce;  // rcap entries
@@ -119,6 +120,7 @@ struct TiledJob {
     size_t off_split = 0, off_tcnt = 0;  // in the arena
     bool convert = false;  // the GCF table is converted to the staged type
     size_t off_gtab = 0;
+    bool taps = false;  // small support: gridded by k_taps
 };
 
 // Record packing of one job (device table, blockIdx.y of k_pack): the job's
@@ -169,7 +171,12 @@ struct TiledPlan {
     bool det = false;   // ordered reduction of split tiles (gather / window)
     int64_t partial_cap = 0, reduce_cap = 0;
     size_t off_partials = 0, off_reduce = 0;
-    bool taps = false;  // small supports: the one-tap-per-lane kernel (k_taps)
+    // small supports go to the one-tap-per-lane kernel (k_taps), with their
+    // own item queue; the other jobs to the owner kernel
+    bool taps = false, owner = true;  // some job uses k_taps / the owner kernel
+    size_t lds_taps = 0;
+    size_t off_items_t = 0, off_qs_t = 0;
+    int64_t item_cap_t = 0;
 };
 
 struct OwnerLaunch {

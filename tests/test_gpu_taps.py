@@ -6,7 +6,8 @@ within 1e-5; BF16 GCF tiles within 1e-5 of the oracle on the bf16-rounded
 table; float pairs within 2e-5.  Against the owner kernel on the same inputs
 (tap_smax = -1 turns the tap kernel off): FP64 within 1e-13.  Dense clusters
 with tiny work items force split tiles (atomic flushes onto zeroed tiles);
-a batch of small supports replays as a graph.  MIGRID_VERBOSE names the
+a batch of small supports replays as a graph, and a batch mixing small and
+large supports runs both kernels, each from its own queue.  MIGRID_VERBOSE names the
 kernel each launch used, which the tests read to be sure which path ran.
 """
 import numpy as np
@@ -193,3 +194,50 @@ def test_small_support_batch_and_graph(capfd, monkeypatch):
                 assert rel(g[..., 0] + 1j * g[..., 1], w) < tol, (mode, j["N"], j["S"])
         b.close()
     assert "tap launch" in capfd.readouterr().err
+
+
+def test_mixed_batch_uses_both_kernels(capfd, monkeypatch):
+    """A batch mixing small and large supports: the small ones go to the tap
+    kernel's queue, the rest to the owner kernel's; two jobs of different
+    supports share one grid (atomic flushes into the caller's zeros)."""
+    t = torch()
+    monkeypatch.setenv("MIGRID_VERBOSE", "1")
+    shapes = [(128, 8, 3000), (200, 40, 4000), (96, 4, 1000), (150, 64, 2000), (180, 16, 5000)]
+    jobs, wants = [], []
+    for i, (N, S, n) in enumerate(shapes):
+        uv, vals = inputs(N, S, n, i % 2 == 0, seed=70 + i)
+        gcf = O.config_gcf(S)
+        wants.append(cplx(O.grid_zz(uv, vals, N, gcf, S)))
+        jobs.append(dict(grid=t.empty((N, N, 2), dtype=t.float64, device="cuda"), uv=dev(uv),
+                         vals=dev(vals), gcf=dev(gcf), N=N, S=S))
+    N = 140
+    shared = t.empty((N, N, 2), dtype=t.float64, device="cuda")
+    want_shared = np.zeros((N, N), complex)
+    for i, S in enumerate((8, 32)):
+        uv, vals = inputs(N, S, 3000, True, seed=90 + i)
+        gcf = O.config_gcf(S)
+        want_shared += cplx(O.grid_zz(uv, vals, N, gcf, S))
+        jobs.append(dict(grid=shared, uv=dev(uv), vals=dev(vals), gcf=dev(gcf), N=N, S=S))
+    ctx = migrid.Context()
+    for arith, tol in ((migrid.ARITH_NATIVE, 1e-12), (migrid.ARITH_F32, 1e-5)):
+        print(f"[mixed batch] arith {arith}", flush=True)
+        b = migrid.Batch(ctx, jobs, arith=arith)
+        stream = t.cuda.Stream()
+        for mode in ("run", "graph", "graph"):
+            for j in jobs[:len(shapes)]:
+                j["grid"].fill_(3.0)
+            shared.zero_()  # jobs sharing a grid add into the caller's grid
+            t.cuda.synchronize()
+            if mode == "run":
+                b.run(stream)
+            else:
+                b.graph_launch(stream)
+            t.cuda.synchronize()
+            for j, w in zip(jobs, wants):
+                g = j["grid"].cpu().numpy()
+                assert rel(g[..., 0] + 1j * g[..., 1], w) < tol, (mode, j["N"], j["S"])
+            g = shared.cpu().numpy()
+            assert rel(g[..., 0] + 1j * g[..., 1], want_shared) < tol, (mode, "shared")
+        b.close()
+    err = capfd.readouterr().err
+    assert "tap launch" in err and "owner launch" in err
