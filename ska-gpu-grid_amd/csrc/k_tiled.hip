@@ -229,8 +229,9 @@ __device__ void zero_tile(TO* grid, int N, int X0, int Y0, int T, bool planar)
 // zeroes its tile when the owner kernel will not store it whole: no items
 // (nothing stores it) or several (they add to it atomically).
 template <typename TO>
-__global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs, int64_t target,
-                                               int planar, Item* __restrict__ items_o,
+__global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs, int64_t target_o,
+                                               int64_t target_t, int planar,
+                                               Item* __restrict__ items_o,
                                                QueueState* __restrict__ qs_o,
                                                Item* __restrict__ items_t,
                                                QueueState* __restrict__ qs_t)
@@ -239,6 +240,7 @@ __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs,
     // the owner kernel's queue, or the small-support kernel's
     Item* __restrict__ items = jobs[job].taps ? items_t : items_o;
     QueueState* __restrict__ qs = jobs[job].taps ? qs_t : qs_o;
+    const int64_t target = jobs[job].taps ? target_t : target_o;
     const TileGeom g = jobs[job].g;
     const int32_t* __restrict__ tcount = jobs[job].tcount;
     uint8_t* __restrict__ split = (uint8_t*)jobs[job].split;
@@ -1240,6 +1242,8 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     }
     p.fresh = !o.accumulate && !p.shared;
     p.target = o.vis_per_wave > 0 ? o.vis_per_wave : 4096;
+    // the tap kernel's items: smaller by default, so dense tiles spread wider
+    p.target_t = o.vis_per_wave > 0 ? o.vis_per_wave : 2048;
     // gather / window: split tiles are summed by an ordered reduction, not
     // atomics.  A tile of c > target visibilities has at most
     // ceil(c / target) < 2c / target items, so 2E / target + 1 partial tiles
@@ -1376,10 +1380,10 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
     Item* items_t = (Item*)(arena + p.off_items_t);
     QueueState* qs_t = (QueueState*)(arena + p.off_qs_t);
     if (p.key.out_type == MIGRID_F64)
-        hipLaunchKernelGGL(k_items<double>, ig, dim3(256), 0, s, ddesc, p.target,
+        hipLaunchKernelGGL(k_items<double>, ig, dim3(256), 0, s, ddesc, p.target, p.target_t,
                            p.key.planar ? 1 : 0, items, qs, items_t, qs_t);
     else
-        hipLaunchKernelGGL(k_items<float>, ig, dim3(256), 0, s, ddesc, p.target,
+        hipLaunchKernelGGL(k_items<float>, ig, dim3(256), 0, s, ddesc, p.target, p.target_t,
                            p.key.planar ? 1 : 0, items, qs, items_t, qs_t);
     MG_HIP(hipGetLastError());
     return MIGRID_OK;

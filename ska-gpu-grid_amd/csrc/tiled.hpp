@@ -163,7 +163,8 @@ struct TiledPlan {
     size_t off_desc = 0, off_items = 0, off_qs = 0, arena_bytes = 0;
     size_t bin_bytes = 0;  // bin-sort scratch
     int64_t item_cap = 0;
-    int64_t target = 0;  // visibilities per work item (0: one item per tile)
+    int64_t target = 0;    // visibilities per work item (0: one item per tile)
+    int64_t target_t = 0;  // the same for the tap kernel's jobs
     int smax = 0;
     size_t lds = 0;
     int stage_min = 0;  // tuning: staging threshold (0: default)
