@@ -241,3 +241,49 @@ def test_mixed_batch_uses_both_kernels(capfd, monkeypatch):
         b.close()
     err = capfd.readouterr().err
     assert "tap launch" in err and "owner launch" in err
+
+
+def sweep_cases(n=24, seed=20261204):
+    rng = np.random.default_rng(seed)
+    cases = []
+    for i in range(n):
+        N = int(rng.choice([3, 7, 16, 33, 64, 129, 300, int(rng.integers(4, 400))]))
+        S = int(2 * rng.integers(1, 9))
+        npts = int(rng.choice([0, 1, 17, 1000, 5000]))
+        dense = bool(rng.random() < 0.4) and npts > 0
+        tile = int(rng.choice([16, 32, 64]))
+        arith = int(rng.choice([migrid.ARITH_NATIVE, migrid.ARITH_F32, migrid.ARITH_F32_BF16GCF]))
+        planar = bool(rng.random() < 0.5)
+        accumulate = bool(rng.random() < 0.3)
+        cases.append((i, N, S, npts, dense, tile, arith, planar, accumulate))
+    return cases
+
+
+@pytest.mark.parametrize("case", sweep_cases(), ids=lambda c: "i%d_N%d_S%d_n%d%s_T%d_a%d%s%s" % (
+    c[0], c[1], c[2], c[3], "_dense" if c[4] else "", c[5], c[6], "_planar" if c[7] else "",
+    "_acc" if c[8] else ""))
+def test_taps_random_sweep(case):
+    """Seeded shapes through the tap kernel: grid edges below the support,
+    empty and single-visibility jobs, dense clusters, every tile edge, the
+    three arithmetic forms, both layouts, fresh and accumulating grids."""
+    i, N, S, npts, dense, tile, arith, planar, accumulate = case
+    print(f"[taps sweep] {case}", flush=True)
+    uv, vals = inputs(N, S, npts, dense, seed=5000 + i)
+    gcf = O.config_gcf(S)
+    table = bf16_round(gcf) if arith == migrid.ARITH_F32_BF16GCF else gcf
+    want = cplx(O.grid_zz(uv, vals, N, table, S))
+    base = np.random.default_rng(i).normal(size=(2, N, N)) if accumulate else None
+    if base is not None and not planar:
+        base = np.moveaxis(base, 0, -1)
+    g = grid_on(uv, vals, gcf, N, S, planar=planar, base=base, strategy="tiled", tile=tile,
+                arith=arith)
+    if base is not None:
+        g = g - (cplx(base) if planar else base[..., 0] + 1j * base[..., 1])
+    tol = 1e-12 if arith == migrid.ARITH_NATIVE else 1e-5
+    m = np.abs(want).max(initial=0.0)
+    if m == 0.0 and base is None:
+        assert np.abs(g).max(initial=0.0) == 0.0
+    else:
+        scale = max(m, 1e-300)
+        extra = 1e-12 * np.abs(base).max() if base is not None else 0.0
+        assert np.abs(g - want).max(initial=0.0) <= tol * scale + extra, f"rel {rel(g, want):.3e}"
