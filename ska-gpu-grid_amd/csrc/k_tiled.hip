@@ -821,11 +821,9 @@ __global__ __launch_bounds__(64 * kTapWaves) __attribute__((amdgpu_waves_per_eu(
         int handed = 0;   // visibilities of the item handed out so far, mod kTapWaves
         for (uint64_t todo = nz_mask; todo; todo &= todo - 1) {
             const int k = __builtin_ctzll(todo);
+            const uint64_t rest = todo & (todo - 1);  // the sub-kernels after k
             P ngr[4], ngi[4];
-            {
-                const uint64_t rest = todo & (todo - 1);
-                if (rest) load_taps(__builtin_ctzll(rest), ngr, ngi);
-            }
+            if (rest) load_taps(__builtin_ctzll(rest), ngr, ngi);
             const bool shared = (big_mask >> k) & 1;
             // lanes 2t and 2t + 1 hold the record range of bin row t
             const int bv = lane < 2 * nrow ? s_bnd[k * kTapRows * 2 + lane] : 0;
@@ -859,10 +857,12 @@ __global__ __launch_bounds__(64 * kTapWaves) __attribute__((amdgpu_waves_per_eu(
                 }
                 if (shared) handed = (handed + (r1 - r0)) & (kTapWaves - 1);
             }
+            if (rest) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                gr[i] = ngr[i];
-                gi[i] = ngi[i];
+                for (int i = 0; i < 4; i++) {
+                    gr[i] = ngr[i];
+                    gi[i] = ngi[i];
+                }
             }
         }
         __syncthreads();
