@@ -28,6 +28,9 @@
 //    from a device queue by persistent workgroups.  A tile that is one item
 //    is flushed with a plain read-add-write; a split tile with hardware
 //    atomics (FP64, FP32, or FP32 with a two-sum compensation grid).
+//  * Small supports (S <= 16, TILED strategy) run the tap kernel k_taps
+//    instead, from their own item queue: lanes are taps of the window rather
+//    than pixels of a block, and the tile is summed in LDS with atomics.
 //
 // Tap semantics as the reference (grid.cu:115-132): tap (r, c) of a
 // visibility with q = main + S/2 lands on pixel (q_x - S + 1 + c,
