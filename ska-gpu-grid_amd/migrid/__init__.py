@@ -186,6 +186,10 @@ def version():
 def make_opts(strategy="scatter", layout=LAYOUT_DEFAULT, gcf_mode=GCF_TABLE, arith=ARITH_NATIVE,
               accumulate=False, debug_coverage=False, tile=0, vis_per_wave=0, sigma=0.0,
               beta=0.05, stage_min=0, wg_per_cu=0, tap_smax=0):
+    """migrid_opts from keywords (fields as in include/migrid.h).  Tiled
+    tuning: tile edge, visibilities per work item, staging threshold,
+    workgroups per CU, and tap_smax, the largest support the tap kernel grids
+    (0: 16, -1: never)."""
     o = Opts()
     lib().migrid_opts_default(ctypes.byref(o))
     o.stage_min = stage_min
