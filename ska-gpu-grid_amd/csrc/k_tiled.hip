@@ -1211,6 +1211,7 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         p.smax = std::max(p.smax, x.S);
     }
     p.owner = smax_owner > 0;
+    p.smax_owner = smax_owner;
     if (p.K >= ((int64_t)1 << 31) || p.E >= ((int64_t)1 << 31))
         return fail(MIGRID_ENOTSUP, "tiled: %lld keys / %lld visibilities in one plan (limit 2^31)",
                     (long long)p.K, (long long)p.E);
@@ -1398,7 +1399,7 @@ int tiled_execute(const TiledPlan& p, char* arena, hipStream_t s)
     L.jobs = (const JobDesc*)(arena + p.off_desc);
     L.items = (const Item*)(arena + p.off_items);
     L.qs = (QueueState*)(arena + p.off_qs);
-    L.smax = p.smax;
+    L.smax = p.smax_owner;  // the owner kernel's LDS layout (p.lds) is sized for it
     L.lds = p.lds;
     L.stage_min = p.stage_min;
     L.wg_per_cu = p.wg_per_cu;

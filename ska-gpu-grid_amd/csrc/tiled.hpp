@@ -175,6 +175,7 @@ struct TiledPlan {
     // small supports go to the one-tap-per-lane kernel (k_taps), with their
     // own item queue; the other jobs to the owner kernel
     bool taps = false, owner = true;  // some job uses k_taps / the owner kernel
+    int smax_owner = 0;               // the largest support among the owner kernel's jobs
     size_t lds_taps = 0;
     size_t off_items_t = 0, off_qs_t = 0;
     int64_t item_cap_t = 0;
