@@ -376,14 +376,15 @@ int run_degrid_v(const DegridArgs& a, size_t lds, int wg_per_cu, hipStream_t s)
     return MIGRID_OK;
 }
 
-// The support-specialised kernels (S = 16, 32, 64 with the automatic tile
-// edge) or the general one.
+// The support-specialised kernels (S = 8, 16, 32, 64 with the automatic
+// tile edge) or the general one.
 template <int V, typename G, typename GS, typename TA, typename TI, typename TO>
 int run_degrid_s(const DegridArgs& a, bool special, size_t lds, int wpc, hipStream_t s)
 {
     constexpr int g = (int)sizeof(GS);
     if (special) {
         switch (a.S) {
+        case 8: return run_degrid_v<V, 8, auto_tile(8, g) + 7, G, GS, TA, TI, TO>(a, lds, wpc, s);
         case 16: return run_degrid_v<V, 16, auto_tile(16, g) + 15, G, GS, TA, TI, TO>(a, lds, wpc, s);
         case 32: return run_degrid_v<V, 32, auto_tile(32, g) + 31, G, GS, TA, TI, TO>(a, lds, wpc, s);
         case 64: return run_degrid_v<V, 64, auto_tile(64, g) + 63, G, GS, TA, TI, TO>(a, lds, wpc, s);
@@ -447,7 +448,7 @@ extern "C" int migrid_ctx_degrid(migrid_ctx* ctx, const migrid_job* job, void* v
         T = o.tile;
     }
     // the specialised kernels assume the automatic tile edge
-    const bool special = o.tile <= 0 && (S == 16 || S == 32 || S == 64);
+    const bool special = o.tile <= 0 && (S == 8 || S == 16 || S == 32 || S == 64);
     if (!T || tile_lds(T) > 160 * 1024)
         return fail(MIGRID_ENOTSUP, "degridding tile does not fit in LDS");
     const size_t lds = tile_lds(T);
