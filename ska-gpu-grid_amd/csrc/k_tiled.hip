@@ -460,9 +460,11 @@ __device__ __forceinline__ void mac4(Cplx<TA> (&acc)[2][2], P vr, P vi, const P 
 }
 
 // WX x WY waves own a (16 WX) x (16 WY) tile; lane (lx, ly) of wave (wx, wy)
-// owns pixels (Xw + lx + 8j, Yw + ly + 8i), i, j in {0, 1}.
+// owns pixels (Xw + lx + 8j, Yw + ly + 8i), i, j in {0, 1}.  SC: every job's
+// support as a compile-time constant (0: read per job), which makes the staged
+// layout's border and pitch, and so the LDS offsets, constants.
 template <int WX, int WY, typename G, typename TA, typename R, typename TI, typename TO,
-          bool PLANAR, int GM>
+          bool PLANAR, int GM, int SC>
 // Register budget: two workgroups per CU need 8 waves per SIMD at T = 64
 // (the 4- and 8-byte staged kernels leave room for two in LDS).
 __global__ __launch_bounds__(64 * WX * WY)
@@ -508,7 +510,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
         const double sigma = jobs[job].sigma, beta = jobs[job].beta;
         const bool atomic = jobs[job].split[tile] != 0;
         const bool fresh = jobs[job].fresh != 0;
-        const int S = g.S;
+        const int S = SC ? SC : g.S;
         const int B = stage_border(S);
         const int pitch = stage_pitch(S);
         const int64_t NN = (int64_t)g.N * g.N;
@@ -961,10 +963,10 @@ int launch_convert(const ConvJob* table, int n, int64_t max_elems, int stage, bo
 
 // ---- host side ------------------------------------------------------------------
 template <int WX, int WY, typename G, typename TA, typename R, typename TI,
-          typename TO, bool PLANAR, int GM>
+          typename TO, bool PLANAR, int GM, int SC = 0>
 int run_owner(const OwnerLaunch& L, hipStream_t s)
 {
-    auto kern = k_owner<WX, WY, G, TA, R, TI, TO, PLANAR, GM>;
+    auto kern = k_owner<WX, WY, G, TA, R, TI, TO, PLANAR, GM, SC>;
     if (L.lds > kMaxLds)
         return fail(MIGRID_ENOTSUP, "tiled: the staged sub-kernel needs %zu bytes of LDS (at most %zu)",
                     L.lds, kMaxLds);
@@ -996,7 +998,11 @@ int run_owner(const OwnerLaunch& L, hipStream_t s)
 template <typename G, typename TA, typename R, typename TI, typename TO, bool PLANAR, int GM>
 int run_shape(int T, const OwnerLaunch& L, hipStream_t s)
 {
-    if (T == 64) return run_owner<4, 4, G, TA, R, TI, TO, PLANAR, GM>(L, s);
+    if (T == 64) {
+        // every job at S = 64 (the large-support case): the support as a constant
+        if (L.sc == 64) return run_owner<4, 4, G, TA, R, TI, TO, PLANAR, GM, 64>(L, s);
+        return run_owner<4, 4, G, TA, R, TI, TO, PLANAR, GM>(L, s);
+    }
     if (T == 32) return run_owner<2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);
     if (T == 16) return run_owner<1, 1, G, TA, R, TI, TO, PLANAR, GM>(L, s);
     return fail(MIGRID_ENOTSUP, "tiled: tile edge %d (16, 32 or 64 supported)", T);
@@ -1134,7 +1140,7 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.jobs.resize(n);
     size_t off = 0;
     int64_t item_cap = 0, item_cap_t = 0;
-    int smax_owner = 0;
+    int smax_owner = 0, smin_owner = 1 << 30;
     // small supports: one tap per lane into an LDS tile (atomic strategy, table GCF)
     const int tap_smax = o.tap_smax == 0 ? kTapMaxS : std::min(o.tap_smax, kTapMaxS);
     for (int j = 0; j < n; j++) {
@@ -1204,14 +1210,17 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         t.taps = o.strategy == MIGRID_TILED && k.gm == 0 && x.S <= tap_smax &&
                  (tc.T + x.S - 2) / 16 + 1 <= kTapRows;
         (t.taps ? item_cap_t : item_cap) += ntiles * kSubKernels;
-        if (t.taps)
+        if (t.taps) {
             p.taps = true;
-        else
+        } else {
             smax_owner = std::max(smax_owner, x.S);
+            smin_owner = std::min(smin_owner, x.S);
+        }
         p.smax = std::max(p.smax, x.S);
     }
     p.owner = smax_owner > 0;
     p.smax_owner = smax_owner;
+    p.s_common = smin_owner == smax_owner ? smax_owner : 0;
     if (p.K >= ((int64_t)1 << 31) || p.E >= ((int64_t)1 << 31))
         return fail(MIGRID_ENOTSUP, "tiled: %lld keys / %lld visibilities in one plan (limit 2^31)",
                     (long long)p.K, (long long)p.E);
@@ -1410,6 +1419,7 @@ int tiled_execute(const TiledPlan& p, char* arena, hipStream_t s)
         Lt.lds = p.lds_taps;
         MG_TRY(dispatch_taps(p.key, Lt, s));
     }
+    L.sc = p.s_common;
     if (p.owner) MG_TRY(dispatch_owner(p.key, L, s));
     if (p.det) {  // the split tiles, summed in item order
         const int nb = (int)std::min<int64_t>(p.reduce_cap, 2048);
