@@ -251,9 +251,9 @@ int migrid_uvw_to_pixels(const double* uvw, int64_t nrow, const double* freq, in
  * 64 * gcf_dim^2 complex values one after another, and gcf_index[n] in
  * [0, ngcf) picks visibility n's table (device int32 array).  The
  * visibilities are grouped by table on the device and gridded as one
- * persistent launch of the owner kernel (strategy TILED, GATHER or WINDOW;
- * SCATTER is taken as TILED), all groups flushing atomically into the one
- * grid.  GCF mode TABLE only.  Synchronises with stream once (to read the
+ * persistent launch of the owner kernel, or of the tap kernel at supports up
+ * to opts.tap_smax under TILED (strategy TILED, GATHER or WINDOW; SCATTER is
+ * taken as TILED), all groups flushing atomically into the one grid.  GCF mode TABLE only.  Synchronises with stream once (to read the
  * group sizes); the gridding itself is asynchronous. */
 int migrid_ctx_grid_indexed(migrid_ctx* ctx, const migrid_job* job, int32_t ngcf,
                             const int32_t* gcf_index, const migrid_opts* opts, void* stream);
