@@ -999,8 +999,9 @@ template <typename G, typename TA, typename R, typename TI, typename TO, bool PL
 int run_shape(int T, const OwnerLaunch& L, hipStream_t s)
 {
     if (T == 64) {
-        // every job at S = 64 (the large-support case): the support as a constant
+        // every job at S = 64 or at S = 32: the support as a constant
         if (L.sc == 64) return run_owner<4, 4, G, TA, R, TI, TO, PLANAR, GM, 64>(L, s);
+        if (L.sc == 32) return run_owner<4, 4, G, TA, R, TI, TO, PLANAR, GM, 32>(L, s);
         return run_owner<4, 4, G, TA, R, TI, TO, PLANAR, GM>(L, s);
     }
     if (T == 32) return run_owner<2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);

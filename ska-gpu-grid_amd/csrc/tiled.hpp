@@ -190,7 +190,7 @@ struct OwnerLaunch {
     int stage_min;  // staging threshold (0: default)
     int wg_per_cu;  // persistent workgroups per CU (0: occupancy limit)
     size_t lds;
-    int sc = 0;  // every job's support when they share one (the owner kernel specialises S = 64)
+    int sc = 0;  // every job's support when they share one (the owner kernel specialises 32, 64)
 };
 
 int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p);
