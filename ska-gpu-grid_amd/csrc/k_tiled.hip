@@ -741,8 +741,9 @@ constexpr int kTapShare = 32;  // visibilities of a sub-kernel in a tile from wh
 constexpr int kTapRows = 5;   // bin rows per tile: (T + S - 2) / 16 + 1 <= 5 at T <= 64, S <= 16
 __host__ __device__ constexpr int tap_pitch(int T) { return T + 4; }  // doubles
 
-// two workgroups per CU: 8 waves per SIMD, at most 64 VGPRs
-template <typename G, typename R, typename TO, bool PLANAR>
+// two workgroups per CU: 8 waves per SIMD, at most 64 VGPRs.  SC: every job's
+// support as a compile-time constant (0: read per job).
+template <typename G, typename R, typename TO, bool PLANAR, int SC>
 __global__ __launch_bounds__(64 * kTapWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_taps(const JobDesc* __restrict__ jobs,
                                                          const Item* __restrict__ items,
                                                          QueueState* __restrict__ qs)
@@ -775,7 +776,7 @@ __global__ __launch_bounds__(64 * kTapWaves) __attribute__((amdgpu_waves_per_eu(
         TO* __restrict__ grid_lo = (TO*)jobs[job].grid_lo;
         const bool atomic = jobs[job].split[tile] != 0;
         const bool fresh = jobs[job].fresh != 0;
-        const int S = g.S, T = g.T, SS = S * S;
+        const int S = SC ? SC : g.S, T = g.T, SS = S * S;
         const int pitch = tap_pitch(T);
         const int plane = T * pitch;
         const int tx = tile % g.ntx, ty = tile / g.ntx;
@@ -1022,10 +1023,10 @@ int run_modes(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
     return run_shape<G, TA, R, TI, TO, false, 2>(k.T, L, s);
 }
 
-template <typename G, typename R, typename TO, bool PLANAR>
+template <typename G, typename R, typename TO, bool PLANAR, int SC>
 int run_taps(const OwnerLaunch& L, hipStream_t s)
 {
-    auto kern = k_taps<G, R, TO, PLANAR>;
+    auto kern = k_taps<G, R, TO, PLANAR, SC>;
     if (L.lds > kMaxLds)
         return fail(MIGRID_ENOTSUP, "tiled: the LDS tile needs %zu bytes (at most %zu)", L.lds,
                     kMaxLds);
@@ -1053,10 +1054,20 @@ int run_taps(const OwnerLaunch& L, hipStream_t s)
     return MIGRID_OK;
 }
 
+template <typename G, typename R, typename TO, bool PLANAR>
+int run_taps_support(const OwnerLaunch& L, hipStream_t s)
+{
+    // every job at S = 8 or at S = 16: the support as a constant
+    if (L.sc == 8) return run_taps<G, R, TO, PLANAR, 8>(L, s);
+    if (L.sc == 16) return run_taps<G, R, TO, PLANAR, 16>(L, s);
+    return run_taps<G, R, TO, PLANAR, 0>(L, s);
+}
+
 template <typename G, typename R, typename TO>
 int run_taps_layout(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
 {
-    return k.planar ? run_taps<G, R, TO, true>(L, s) : run_taps<G, R, TO, false>(L, s);
+    return k.planar ? run_taps_support<G, R, TO, true>(L, s)
+                    : run_taps_support<G, R, TO, false>(L, s);
 }
 
 template <typename TI, typename R>
@@ -1141,7 +1152,7 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.jobs.resize(n);
     size_t off = 0;
     int64_t item_cap = 0, item_cap_t = 0;
-    int smax_owner = 0, smin_owner = 1 << 30;
+    int smax_owner = 0, smin_owner = 1 << 30, smax_t = 0, smin_t = 1 << 30;
     // small supports: one tap per lane into an LDS tile (atomic strategy, table GCF)
     const int tap_smax = o.tap_smax == 0 ? kTapMaxS : std::min(o.tap_smax, kTapMaxS);
     for (int j = 0; j < n; j++) {
@@ -1213,6 +1224,8 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         (t.taps ? item_cap_t : item_cap) += ntiles * kSubKernels;
         if (t.taps) {
             p.taps = true;
+            smax_t = std::max(smax_t, x.S);
+            smin_t = std::min(smin_t, x.S);
         } else {
             smax_owner = std::max(smax_owner, x.S);
             smin_owner = std::min(smin_owner, x.S);
@@ -1222,6 +1235,7 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.owner = smax_owner > 0;
     p.smax_owner = smax_owner;
     p.s_common = smin_owner == smax_owner ? smax_owner : 0;
+    p.s_common_t = smin_t == smax_t ? smax_t : 0;
     if (p.K >= ((int64_t)1 << 31) || p.E >= ((int64_t)1 << 31))
         return fail(MIGRID_ENOTSUP, "tiled: %lld keys / %lld visibilities in one plan (limit 2^31)",
                     (long long)p.K, (long long)p.E);
@@ -1418,6 +1432,7 @@ int tiled_execute(const TiledPlan& p, char* arena, hipStream_t s)
         Lt.items = (const Item*)(arena + p.off_items_t);
         Lt.qs = (QueueState*)(arena + p.off_qs_t);
         Lt.lds = p.lds_taps;
+        Lt.sc = p.s_common_t;
         MG_TRY(dispatch_taps(p.key, Lt, s));
     }
     L.sc = p.s_common;
