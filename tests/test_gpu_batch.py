@@ -124,3 +124,34 @@ def test_staged_and_direct_paths(stage_min, arith, tol):
                  stage_min=stage_min, wg_per_cu=wg)
         t.cuda.synchronize()
         assert O.rel_errors(cplx(grid), ref)[0] < tol
+
+
+@pytest.mark.parametrize("S", [8, 16, 32, 64])
+def test_batch_of_one_support(S):
+    """Jobs that share one support take the kernels specialised for it (tap
+    kernel at S = 8 and 16, owner kernel at S = 32 and 64); grids of
+    different sizes, a graph replay after a direct run."""
+    t = torch()
+    print(f"[batch one support] S={S}", flush=True)
+    jobs, refs = [], []
+    for i, (N, n) in enumerate([(96, 2000), (150, 4000), (64, 500)]):
+        if i == 1:
+            uv, vals = O.gen_clustered(n, N, S, mean_r=N / 8, seed=90 + i)
+        else:
+            uv, vals = O.gen_uniform(n, -S / 2, N + S / 2, seed=90 + i)
+        gcf = O.config_gcf(S)
+        refs.append(cplx_np(O.grid_zz(uv, vals, N, gcf, S)))
+        jobs.append(dict(grid=t.empty((N, N, 2), dtype=t.float64, device="cuda"), uv=dev(uv),
+                         vals=dev(vals), gcf=dev(gcf), N=N, S=S))
+    ctx = migrid.Context()
+    b = migrid.Batch(ctx, jobs)
+    stream = t.cuda.Stream()
+    for mode in ("run", "graph"):
+        for j in jobs:
+            j["grid"].fill_(5.0)
+        t.cuda.synchronize()
+        b.run(stream) if mode == "run" else b.graph_launch(stream)
+        t.cuda.synchronize()
+        for j, r in zip(jobs, refs):
+            assert O.rel_errors(cplx(j["grid"]), r)[0] < 1e-12, (mode, j["N"])
+    b.close()
