@@ -166,7 +166,7 @@ struct TiledPlan {
     int64_t target = 0;    // visibilities per work item (0: one item per tile)
     int64_t target_t = 0;  // the same for the tap kernel's jobs
     int smax = 0;
-    size_t lds = 0;
+    size_t lds = 0;     // the owner kernel's dynamic LDS (its staged sub-kernel layout)
     int stage_min = 0;  // tuning: staging threshold (0: default)
     int wg_per_cu = 0;  // tuning: persistent workgroups per CU (0: occupancy)
     bool det = false;   // ordered reduction of split tiles (gather / window)
@@ -203,7 +203,9 @@ int tiled_write_descs(const TiledPlan& p, char* arena, const std::vector<JobDesc
 // per-tile counts and work items (one launch each).  For fresh grids the
 // work-item pass zeroes the tiles the owner kernel will not store whole.
 int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_t s);
-// The persistent owner kernel over all queued items.
+// The persistent kernels over their queued items: the tap kernel for the
+// small-support jobs, the owner kernel for the rest, then (gather / window)
+// the ordered sum of split tiles.
 int tiled_execute(const TiledPlan& p, char* arena, hipStream_t s);
 int dispatch_owner(const KernelKey& k, const OwnerLaunch& L, hipStream_t s);
 // The small-support kernel (one tap per lane, LDS tile) over all queued items.
