@@ -741,9 +741,11 @@ constexpr int kTapShare = 32;  // visibilities of a sub-kernel in a tile from wh
 constexpr int kTapRows = 5;   // bin rows per tile: (T + S - 2) / 16 + 1 <= 5 at T <= 64, S <= 16
 __host__ __device__ constexpr int tap_pitch(int T) { return T + 4; }  // doubles
 
-// two workgroups per CU: 8 waves per SIMD, at most 64 VGPRs.  SC: every job's
-// support as a compile-time constant (0: read per job).
-template <typename G, typename R, typename TO, bool PLANAR, int SC>
+// two workgroups per CU: 8 waves per SIMD, at most 64 VGPRs.  GM: 0 table
+// GCF, 1 constant tap (COMPUTE_GCF and the coverage map: every tap 1, the
+// record carries the transformed value).  SC: every job's support as a
+// compile-time constant (0: read per job).
+template <typename G, typename R, typename TO, bool PLANAR, int GM, int SC>
 __global__ __launch_bounds__(64 * kTapWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_taps(const JobDesc* __restrict__ jobs,
                                                          const Item* __restrict__ items,
                                                          QueueState* __restrict__ qs)
@@ -817,8 +819,13 @@ __global__ __launch_bounds__(64 * kTapWaves) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 gr[i] = gi[i] = (P)0;
-                if (i < ntap && lane + 64 * i < SS)
-                    Stage<G>::get(gload<G>(gtab, (int64_t)kk * SS + lane + 64 * i), gr[i], gi[i]);
+                if (i < ntap && lane + 64 * i < SS) {
+                    if (GM == 1)
+                        gr[i] = (P)1;
+                    else
+                        Stage<G>::get(gload<G>(gtab, (int64_t)kk * SS + lane + 64 * i), gr[i],
+                                      gi[i]);
+                }
             }
         };
         P gr[4], gi[4];
@@ -1023,10 +1030,10 @@ int run_modes(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
     return run_shape<G, TA, R, TI, TO, false, 2>(k.T, L, s);
 }
 
-template <typename G, typename R, typename TO, bool PLANAR, int SC>
+template <typename G, typename R, typename TO, bool PLANAR, int GM, int SC>
 int run_taps(const OwnerLaunch& L, hipStream_t s)
 {
-    auto kern = k_taps<G, R, TO, PLANAR, SC>;
+    auto kern = k_taps<G, R, TO, PLANAR, GM, SC>;
     if (L.lds > kMaxLds)
         return fail(MIGRID_ENOTSUP, "tiled: the LDS tile needs %zu bytes (at most %zu)", L.lds,
                     kMaxLds);
@@ -1055,19 +1062,20 @@ int run_taps(const OwnerLaunch& L, hipStream_t s)
 }
 
 template <typename G, typename R, typename TO, bool PLANAR>
-int run_taps_support(const OwnerLaunch& L, hipStream_t s)
+int run_taps_support(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
 {
-    // every job at S = 8 or at S = 16: the support as a constant
-    if (L.sc == 8) return run_taps<G, R, TO, PLANAR, 8>(L, s);
-    if (L.sc == 16) return run_taps<G, R, TO, PLANAR, 16>(L, s);
-    return run_taps<G, R, TO, PLANAR, 0>(L, s);
+    if (k.gm == 1) return run_taps<G, R, TO, PLANAR, 1, 0>(L, s);  // constant tap
+    // a table GCF with every job at S = 8 or at S = 16: the support as a constant
+    if (L.sc == 8) return run_taps<G, R, TO, PLANAR, 0, 8>(L, s);
+    if (L.sc == 16) return run_taps<G, R, TO, PLANAR, 0, 16>(L, s);
+    return run_taps<G, R, TO, PLANAR, 0, 0>(L, s);
 }
 
 template <typename G, typename R, typename TO>
 int run_taps_layout(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
 {
-    return k.planar ? run_taps_support<G, R, TO, true>(L, s)
-                    : run_taps_support<G, R, TO, false>(L, s);
+    return k.planar ? run_taps_support<G, R, TO, true>(k, L, s)
+                    : run_taps_support<G, R, TO, false>(k, L, s);
 }
 
 template <typename TI, typename R>
@@ -1117,7 +1125,8 @@ int dispatch_owner(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
 
 int dispatch_taps(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
 {
-    if (k.gm != 0) return fail(MIGRID_ENOTSUP, "tap kernel: table GCF only");
+    if (k.gm != 0 && k.gm != 1)
+        return fail(MIGRID_ENOTSUP, "tap kernel: table or constant-tap GCF only");
     const bool out64 = k.out_type == MIGRID_F64;
     switch (k.stage) {
     case kStageBf16:
@@ -1153,7 +1162,8 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     size_t off = 0;
     int64_t item_cap = 0, item_cap_t = 0;
     int smax_owner = 0, smin_owner = 1 << 30, smax_t = 0, smin_t = 1 << 30;
-    // small supports: one tap per lane into an LDS tile (atomic strategy, table GCF)
+    // small supports: one tap per lane into an LDS tile (atomic strategy; table
+    // or constant-tap GCF)
     const int tap_smax = o.tap_smax == 0 ? kTapMaxS : std::min(o.tap_smax, kTapMaxS);
     for (int j = 0; j < n; j++) {
         const GridArgs& x = a[j];
@@ -1219,7 +1229,7 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
             p.nconv++;
             p.max_gcf = std::max<int64_t>(p.max_gcf, (int64_t)kSubKernels * x.S * x.S);
         }
-        t.taps = o.strategy == MIGRID_TILED && k.gm == 0 && x.S <= tap_smax &&
+        t.taps = o.strategy == MIGRID_TILED && k.gm != 2 && x.S <= tap_smax &&
                  (tc.T + x.S - 2) / 16 + 1 <= kTapRows;
         (t.taps ? item_cap_t : item_cap) += ntiles * kSubKernels;
         if (t.taps) {

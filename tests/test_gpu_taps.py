@@ -287,3 +287,32 @@ def test_taps_random_sweep(case):
         scale = max(m, 1e-300)
         extra = 1e-12 * np.abs(base).max() if base is not None else 0.0
         assert np.abs(g - want).max(initial=0.0) <= tol * scale + extra, f"rel {rel(g, want):.3e}"
+
+
+@pytest.mark.parametrize("S", [8, 16])
+def test_taps_coverage_bit_exact(S, capfd, monkeypatch):
+    """The DEBUG1 coverage map (every tap adds (1, n)) through the tap kernel:
+    sums of integers in double, equal to the oracle's bit for bit."""
+    monkeypatch.setenv("MIGRID_VERBOSE", "1")
+    N = 120
+    uv, vals = O.gen_uniform(3000, -S, N + S, seed=S + 3)
+    ref = O.grid_zz(uv, vals, N, O.config_gcf(S), S, debug1=True)
+    g = grid_on(uv, vals, O.config_gcf(S), N, S, strategy="tiled", debug_coverage=True)
+    np.testing.assert_array_equal(g.real, ref[0])
+    np.testing.assert_array_equal(g.imag, ref[1])
+    assert "tap launch" in capfd.readouterr().err
+
+
+@pytest.mark.parametrize("S", [4, 8, 16])
+def test_taps_compute_gcf_matches_owner(S, capfd, monkeypatch):
+    """COMPUTE_GCF (the constant-tap form, values carrying the phase) on the
+    tap kernel against the owner kernel on the same inputs."""
+    monkeypatch.setenv("MIGRID_VERBOSE", "1")
+    N = 140
+    uv, vals = inputs(N, S, 4000, True, seed=S + 11)
+    gcf = O.init_gcf(S)
+    a = grid_on(uv, vals, gcf, N, S, strategy="tiled", gcf_mode=migrid.GCF_COMPAT)
+    b = grid_on(uv, vals, gcf, N, S, strategy="tiled", gcf_mode=migrid.GCF_COMPAT, tap_smax=-1)
+    err = capfd.readouterr().err
+    assert err.count("tap launch") == 1 and err.count("owner launch") == 1
+    assert rel(a, b) < 1e-13
