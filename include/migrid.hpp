@@ -14,7 +14,9 @@
  * The reference chooses its strategy with build flags (Makefile:36-47); this
  * header reads the same macros: __GATHER -> gather, __MOVING_WINDOW -> moving
  * window (planar output), otherwise scatter; __COMPUTE_GCF -> the reference's
- * on-the-fly GCF.  Errors are printed and execution continues, as the
+ * on-the-fly GCF.  One flag of its own, MIGRID_USE_TILED, picks the tiled
+ * strategy (the fastest, with no counterpart in the reference) where the
+ * reference would scatter.  Errors are printed and execution continues, as the
  * reference's CUDA_CHECK_ERR does (grid_gpu.cu:32-36); use the C ABI to get
  * status codes instead.
  *
@@ -40,6 +42,8 @@ inline migrid_opts build_flag_opts()
     o.strategy = MIGRID_GATHER;
 #elif defined(__MOVING_WINDOW)
     o.strategy = MIGRID_WINDOW;
+#elif defined(MIGRID_USE_TILED)
+    o.strategy = MIGRID_TILED;
 #else
     o.strategy = MIGRID_SCATTER;
 #endif
