@@ -46,7 +46,7 @@ typedef struct { float x, y; } migrid_c64;   /* layout of float2 */
  * grid per job; jobs of a batch sharing a grid flush atomically).  TILED is
  * the bin-sorted LDS-tiled gridder with atomics only where a split tile is
  * flushed; at supports up to opts.tap_smax (16 by default) it sums each tile
- * one tap per lane with LDS atomics instead (the tap kernel), 1.6-5x faster
+ * one tap per lane with LDS atomics instead (the tap kernel), 2-5.7x faster
  * there. */
 #define MIGRID_SCATTER 0
 #define MIGRID_GATHER 1
