@@ -743,8 +743,8 @@ __host__ __device__ constexpr int tap_pitch(int T) { return T + 4; }  // doubles
 
 // two workgroups per CU: 8 waves per SIMD, at most 64 VGPRs.  GM: 0 table
 // GCF, 1 constant tap (COMPUTE_GCF and the coverage map: every tap 1, the
-// record carries the transformed value).  SC: every job's support as a
-// compile-time constant (0: read per job).
+// record carries the transformed value), 2 analytic GCF.  SC: every job's
+// support as a compile-time constant (0: read per job).
 template <typename G, typename R, typename TO, bool PLANAR, int GM, int SC>
 __global__ __launch_bounds__(64 * kTapWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_taps(const JobDesc* __restrict__ jobs,
                                                          const Item* __restrict__ items,
@@ -755,6 +755,7 @@ __global__ __launch_bounds__(64 * kTapWaves) __attribute__((amdgpu_waves_per_eu(
     double* tacc = (double*)smem_raw;
     __shared__ int s_item;
     __shared__ int s_bnd[kSubKernels * kTapRows * 2];  // per sub-kernel and bin row: record range
+    __shared__ Cplx<double> s_fac[2][kOversample][kTapMaxS];  // analytic factors (GM == 2)
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -815,22 +816,46 @@ __global__ __launch_bounds__(64 * kTapWaves) __attribute__((amdgpu_waves_per_eu(
             const uint64_t mine = __ballot(((small_all >> lane) & 1) && (rank & (kTapWaves - 1)) == w);
             nz_mask = big_mask | mine;
         }
+        if (GM == 2) {
+            // separable analytic GCF, as the owner kernel stages it: G[r][c] =
+            // f_y(r) f_x(c), f(d) = exp(-d^2/(2 sigma^2)) exp(i beta d^2) with d
+            // the tap's offset from the visibility for each sub-pixel position
+            const double sigma = jobs[job].sigma, beta = jobs[job].beta;
+            for (int e = tid; e < 2 * kOversample * S; e += blockDim.x) {
+                const int axis = e / (kOversample * S), sp = (e / S) % kOversample, t = e % S;
+                const double d = (double)(t - S / 2 + 1) - sp / 8.0;
+                const double q = d * d;
+                const double amp = exp(-q / (2.0 * sigma * sigma));
+                double sn, cs;
+                sincos(beta * q, &sn, &cs);
+                s_fac[axis][sp][t].re = amp * cs;
+                s_fac[axis][sp][t].im = amp * sn;
+            }
+        }
         auto load_taps = [&](int kk, P(&gr)[4], P(&gi)[4]) {
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 gr[i] = gi[i] = (P)0;
                 if (i < ntap && lane + 64 * i < SS) {
-                    if (GM == 1)
+                    if (GM == 1) {
                         gr[i] = (P)1;
-                    else
+                    } else if (GM == 2) {
+                        const Cplx<double> a = s_fac[1][kk / kOversample][tr[i]];
+                        const Cplx<double> b = s_fac[0][kk % kOversample][tc[i]];
+                        Stage<G>::get(Stage<G>::make(a.re * b.re - a.im * b.im,
+                                                     a.re * b.im + a.im * b.re),
+                                      gr[i], gi[i]);
+                    } else {
                         Stage<G>::get(gload<G>(gtab, (int64_t)kk * SS + lane + 64 * i), gr[i],
                                       gi[i]);
+                    }
                 }
             }
         };
         P gr[4], gi[4];
-        if (nz_mask) load_taps(__builtin_ctzll(nz_mask), gr, gi);
-        __syncthreads();  // tile cleared, bounds listed
+        if (GM != 2 && nz_mask) load_taps(__builtin_ctzll(nz_mask), gr, gi);
+        __syncthreads();  // tile cleared, bounds listed (and the analytic factors)
+        if (GM == 2 && nz_mask) load_taps(__builtin_ctzll(nz_mask), gr, gi);
         int handed = 0;   // visibilities of the item handed out so far, mod kTapWaves
         for (uint64_t todo = nz_mask; todo; todo &= todo - 1) {
             const int k = __builtin_ctzll(todo);
@@ -1065,6 +1090,7 @@ template <typename G, typename R, typename TO, bool PLANAR>
 int run_taps_support(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
 {
     if (k.gm == 1) return run_taps<G, R, TO, PLANAR, 1, 0>(L, s);  // constant tap
+    if (k.gm == 2) return run_taps<G, R, TO, PLANAR, 2, 0>(L, s);  // analytic
     // a table GCF with every job at S = 8 or at S = 16: the support as a constant
     if (L.sc == 8) return run_taps<G, R, TO, PLANAR, 0, 8>(L, s);
     if (L.sc == 16) return run_taps<G, R, TO, PLANAR, 0, 16>(L, s);
@@ -1125,8 +1151,6 @@ int dispatch_owner(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
 
 int dispatch_taps(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
 {
-    if (k.gm != 0 && k.gm != 1)
-        return fail(MIGRID_ENOTSUP, "tap kernel: table or constant-tap GCF only");
     const bool out64 = k.out_type == MIGRID_F64;
     switch (k.stage) {
     case kStageBf16:
@@ -1162,8 +1186,7 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     size_t off = 0;
     int64_t item_cap = 0, item_cap_t = 0;
     int smax_owner = 0, smin_owner = 1 << 30, smax_t = 0, smin_t = 1 << 30;
-    // small supports: one tap per lane into an LDS tile (atomic strategy; table
-    // or constant-tap GCF)
+    // small supports: one tap per lane into an LDS tile (atomic strategy)
     const int tap_smax = o.tap_smax == 0 ? kTapMaxS : std::min(o.tap_smax, kTapMaxS);
     for (int j = 0; j < n; j++) {
         const GridArgs& x = a[j];
@@ -1229,7 +1252,7 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
             p.nconv++;
             p.max_gcf = std::max<int64_t>(p.max_gcf, (int64_t)kSubKernels * x.S * x.S);
         }
-        t.taps = o.strategy == MIGRID_TILED && k.gm != 2 && x.S <= tap_smax &&
+        t.taps = o.strategy == MIGRID_TILED && x.S <= tap_smax &&
                  (tc.T + x.S - 2) / 16 + 1 <= kTapRows;
         (t.taps ? item_cap_t : item_cap) += ntiles * kSubKernels;
         if (t.taps) {

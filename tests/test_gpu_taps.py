@@ -316,3 +316,26 @@ def test_taps_compute_gcf_matches_owner(S, capfd, monkeypatch):
     err = capfd.readouterr().err
     assert err.count("tap launch") == 1 and err.count("owner launch") == 1
     assert rel(a, b) < 1e-13
+
+
+@pytest.mark.parametrize("S", [8, 16])
+@pytest.mark.parametrize("arith,tol", [(migrid.ARITH_NATIVE, 1e-13), (migrid.ARITH_F32, 1e-5)])
+def test_taps_analytic_gcf_matches_owner(S, arith, tol, capfd, monkeypatch):
+    """The analytic GCF (Gaussian x chirp, evaluated on the device) on the tap
+    kernel against the owner kernel, which stages the same factors."""
+    monkeypatch.setenv("MIGRID_VERBOSE", "1")
+    t = torch()
+    N = 130
+    uv, vals = inputs(N, S, 3000, False, seed=S + 23)
+    grids = []
+    for tap_smax in (0, -1):
+        g = t.empty((N, N, 2), dtype=t.float64, device="cuda")
+        migrid.Context().grid(g, dev(uv), dev(vals), None, N, S, strategy="tiled",
+                              gcf_mode=migrid.GCF_ANALYTIC, sigma=S / 6.0, beta=0.05,
+                              arith=arith, tap_smax=tap_smax)
+        t.cuda.synchronize()
+        h = g.cpu().numpy()
+        grids.append(h[..., 0] + 1j * h[..., 1])
+    err = capfd.readouterr().err
+    assert "tap launch" in err and "owner launch" in err
+    assert rel(grids[0], grids[1]) < tol
