@@ -737,7 +737,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
 // fixed order, so this kernel serves the atomic TILED strategy.
 constexpr int kTapMaxS = 16;  // at most 4 taps per lane
 constexpr int kTapWaves = 16;  // a power of two (round-robin by mask)
-constexpr int kTapShare = 32;  // visibilities of a sub-kernel in a tile from which all waves share it
+constexpr int kTapShare = 64;  // visibilities of a sub-kernel in a tile from which all waves share it
 constexpr int kTapRows = 5;   // bin rows per tile: (T + S - 2) / 16 + 1 <= 5 at T <= 64, S <= 16
 __host__ __device__ constexpr int tap_pitch(int T) { return T + 4; }  // doubles
 
