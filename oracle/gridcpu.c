@@ -251,9 +251,9 @@ void oracle_grid_zc(double* out, const float* uv, const float* vals, int64_t npt
 }
 
 /* PRECISION=float, OUTPRECISION=float: the same with a float planar grid.
- * (The reference's gridCPU_pz casts a float2 grid to double* here, K1 in the
- * survey; the restatement keeps float storage, which is what the GPU pair
- * <float2,float2> produces.) */
+ * (The reference's gridCPU_pz casts its output to double* whatever its type,
+ * grid.cu:153-154, so a float2 grid is written as doubles; the restatement
+ * keeps float storage, which is what the GPU pair <float2,float2> produces.) */
 void oracle_grid_cc(float* out, const float* uv, const float* vals, int64_t npts,
                     int64_t N, const float* gcf, int64_t S, int zero_first)
 {
@@ -284,8 +284,9 @@ void oracle_grid_cc(float* out, const float* uv, const float* vals, int64_t npts
     }
 }
 
-/* Number of visibilities whose floorf-emulated sub-pixel index differs from
- * the exact one (survey finding K4; zero for float-representable uv). */
+/* Number of coordinates whose gridCPU-form sub-pixel index (floorf on
+ * float-rounded values, grid.cu:107-111) differs from the exact double one
+ * the GPU kernels use (grid_gpu.cu:118-121); zero for float-representable uv. */
 int64_t oracle_floorf_divergent(const double* uv, int64_t npts)
 {
     int64_t cnt = 0;
