@@ -125,30 +125,75 @@ def time_variant(torch, ctx, fn, warmup, steps):
     return wall, per, kern
 
 
-def cpu_baseline(uv, vals, gcf, N, S, nsample):
+def host_cpu():
+    """The host's CPU model and the cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": len(os.sched_getaffinity(0)),
+            "logical_cpus": os.cpu_count()}
+
+
+def omp_threads():
+    """OMP_NUM_THREADS, else the usable cores up to 16 (the box's CPU share)."""
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, len(os.sched_getaffinity(0)))
+
+
+def oracle_mod():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O  # the CPU restatement of the reference gridCPU
+    import oracle as O  # the CPU restatement of the reference gridCPU (the checker)
+    return O
+
+
+def cpu_baseline(uv, vals, gcf, N, S, nsample, workload):
+    """The reference's own parallelism: gridCPU (grid.cu:87-139) on one core,
+    in its loop order, over a bounded sample of the workload."""
+    O = oracle_mod()
     t0 = time.perf_counter()
     O.grid_zz(uv[:nsample], vals[:nsample], N, gcf, S)
     dt = time.perf_counter() - t0
-    return {"value": nsample / dt / 1e6, "unit": "Mvis/s", "cores": 1, "kind": "port",
-            "sample": f"first {nsample} visibilities of the same 8192^2/S=64 workload through the "
-                      f"single-threaded gridCPU restatement (reference loop order), {dt:.1f} s"}
+    return dict({"value": nsample / dt / 1e6, "unit": "Mvis/s", "cores": 1, "kind": "port",
+                 "sample": f"first {nsample} visibilities of the {workload} workload through the "
+                           f"single-threaded gridCPU restatement (reference loop order), {dt:.1f} s"},
+                **host_cpu())
 
 
-def cpu_baseline_threads(uv, vals, gcf, N, S, nsample):
-    """The same restatement split into row bands over the host's threads
-    (OMP_NUM_THREADS, else up to 16): a stronger CPU figure beside the
-    single-threaded one."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    want = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+def oracle_full(uv, vals, gcf, N, S, workload, r2=None, i2=None):
+    """The whole workload through the row-band OpenMP form of the gridCPU
+    restatement (bit-identical to the serial loop): the error oracle of every
+    variant, and the threaded CPU figure.  With r2 / i2, the constant-tap
+    (COMPUTE_GCF) form.  Returns (planar grid, baseline dict)."""
+    O = oracle_mod()
+    want = omp_threads()
     t0 = time.perf_counter()
-    _, used = O.grid_zz_omp(uv[:nsample], vals[:nsample], N, gcf, S, want)
+    if r2 is None:
+        ref, used = O.grid_zz_omp(uv, vals, N, gcf, S, want)
+    else:
+        ref, used = O.grid_const_tap_omp(uv, vals, N, S, r2, i2, want)
     dt = time.perf_counter() - t0
-    return {"value": nsample / dt / 1e6, "unit": "Mvis/s", "cores": int(used), "kind": "port",
-            "sample": f"first {nsample} visibilities of the same workload, row-band OpenMP form "
-                      f"of the gridCPU restatement on {used} threads, {dt:.1f} s"}
+    n = uv.shape[0]
+    return ref, dict({"value": n / dt / 1e6, "unit": "Mvis/s", "cores": int(used), "kind": "port",
+                      "sample": f"the whole {workload} workload ({n} visibilities), row-band "
+                                f"OpenMP form of the gridCPU restatement on {used} threads, "
+                                f"{dt:.1f} s"}, **host_cpu())
+
+
+# COMPUTE_GCF reads T and w from the table element S(S-1)/2-1
+# (grid_gpu.cu:103-108); the benchmark GCF holds 1 + 0i there (w = 0, a
+# constant tap), so the computed variants use a copy holding T = 2, w = 0.31.
+COMPAT_T, COMPAT_W = 2.0, 0.31
+
+
+def compat_table(gcf, S):
+    g = np.array(gcf, np.float64, copy=True)
+    g[S * (S - 1) // 2 - 1] = (COMPAT_T, COMPAT_W)
+    return g
 
 
 def gen_clustered_dev(torch, npts, N, S, mean_r, gen):
@@ -485,10 +530,12 @@ def main():
     ap.add_argument("--npts", type=int, default=10_000_000)
     ap.add_argument("--N", type=int, default=8192)
     ap.add_argument("--S", type=int, default=64)
-    ap.add_argument("--cpu-sample", type=int, default=40000)
+    ap.add_argument("--cpu-sample", type=int, default=100000,
+                    help="visibilities of the single-threaded CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variants", default="fp64_plain,fp64_tiled,fp32_tiled,bf16_tiled,fp64_gather,"
-                                          "fp64_window,fp64_plain_computed,fp64_tiled_computed")
+    ap.add_argument("--variants", default="fp64_plain,fp64_tiled,fp32_tiled,fp32_tiled_twosum,"
+                                          "bf16_tiled,fp64_gather,fp64_window,fp64_plain_computed,"
+                                          "fp64_tiled_computed")
     ap.add_argument("--tile", type=int, default=0, help="tiled: tile edge 16/32/64 (0 auto)")
     ap.add_argument("--vpw", type=int, default=0, help="tiled: visibilities per work item")
     ap.add_argument("--stage-min", type=int, default=0, help="tiled: staging threshold")
@@ -496,12 +543,12 @@ def main():
     ap.add_argument("--tap-smax", type=int, default=0,
                     help="tiled: largest support gridded one tap per lane (0: 16, -1: never)")
     ap.add_argument("--dist", default="clustered", choices=["clustered", "uniform"])
-    ap.add_argument("--mode", default="grid", choices=["grid", "batch", "serve", "copy", "degrid", "image"],
+    ap.add_argument("--mode", default="grid", choices=["grid", "batch", "serve", "copy", "degrid", "image", "cpu"],
                     help="grid: one 8192^2 job (default); batch: many jobs in one persistent "
                          "launch; serve: Poisson request stream through the serving loop; "
                          "copy: 1 GiB device copy (byte-counter calibration); degrid: the "
                          "adjoint (grid -> visibilities) of the grid workload; image: the "
-                         "grid -> image FFT")
+                         "grid -> image FFT; cpu: the CPU baselines in full (no GPU)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--jobs", type=int, default=64, help="batch/serve: job ids")
     ap.add_argument("--large-jobs", type=int, default=4, help="serve: 8192^2 job ids")
@@ -517,6 +564,8 @@ def main():
                                    "(multi-GPU sharding is described, not built)" % args.gpus}))
         sys.exit(2)
 
+    if args.mode == "cpu":
+        return main_cpu(args)
     import torch
     import migrid
 
@@ -534,63 +583,94 @@ def main():
     t0 = time.perf_counter()
     uv, vals = gen_clustered(npts, N, S) if args.dist == "clustered" else gen_uniform(npts, N, S)
     gcf = config_gcf(S)
+    gcf_c = compat_table(gcf, S)
     gen_s = time.perf_counter() - t0
     print(f"# generated {npts} visibilities in {gen_s:.1f} s", file=sys.stderr, flush=True)
+    workload = f"{N}^2/S={S}/{npts} {args.dist}"
 
     d_uv = torch.from_numpy(uv).cuda()
     d_vals = torch.from_numpy(vals).cuda()
     d_gcf = torch.from_numpy(gcf).cuda()
+    d_gcf_c = torch.from_numpy(gcf_c).cuda()
     grid = torch.empty((N, N, 2), dtype=torch.float64, device="cuda")
+    grid32 = grid_lo = None
     ctx = migrid.Context()
 
     variants = {
         "fp64_plain": dict(strategy="scatter"),
         "fp64_tiled": dict(strategy="tiled", arith=migrid.ARITH_NATIVE),
         "fp32_tiled": dict(strategy="tiled", arith=migrid.ARITH_F32),
+        # FP32 tiles flushed into an FP32 grid plus its two-sum compensation grid
+        "fp32_tiled_twosum": dict(strategy="tiled", arith=migrid.ARITH_F32, out32=True),
         "bf16_tiled": dict(strategy="tiled", arith=migrid.ARITH_F32_BF16GCF),
         # the reference's other two strategies, FP64 (interleaved for the comparison)
         "fp64_gather": dict(strategy="gather"),
         "fp64_window": dict(strategy="window", layout=migrid.LAYOUT_INTERLEAVED),
-        # COMPUTE_GCF: the phase form evaluated per tap, scatter and tiled
+        # COMPUTE_GCF: the phase form evaluated per visibility, scatter and tiled
         "fp64_plain_computed": dict(strategy="scatter", gcf_mode=migrid.GCF_COMPAT),
         "fp64_tiled_computed": dict(strategy="tiled", gcf_mode=migrid.GCF_COMPAT),
     }
     tune = dict(tile=args.tile, vis_per_wave=args.vpw, stage_min=args.stage_min,
                 wg_per_cu=args.wg_per_cu, tap_smax=args.tap_smax)
+    names = args.variants.split(",")
+    # the FP64 oracle of the whole workload, on the host's cores (also the
+    # threaded CPU figure); the constant-tap oracle for the COMPUTE_GCF variants
+    oracle = {}
+    if not args.no_cpu:
+        print(f"# oracle: gridCPU restatement over the whole workload on {omp_threads()} threads",
+              file=sys.stderr, flush=True)
+        ref, base_thr = oracle_full(uv, vals, gcf, N, S, workload)
+        oracle["table"] = torch.from_numpy(np.ascontiguousarray(np.moveaxis(ref, 0, -1))).cuda()
+        del ref
+        print(f"# oracle: {base_thr['sample']}", file=sys.stderr, flush=True)
+        if any(variants[n].get("gcf_mode") == migrid.GCF_COMPAT for n in names):
+            O = oracle_mod()
+            r2, i2 = O.compute_gcf_phase(uv, COMPAT_T, COMPAT_W, fixed_point=False)
+            ref, _ = oracle_full(uv, vals, gcf, N, S, workload, r2, i2)
+            oracle["compat"] = torch.from_numpy(np.ascontiguousarray(np.moveaxis(ref, 0, -1))).cuda()
+            del ref, r2, i2
     results = {}
-    ref_grid = None       # the plain FP64 scatter's grid (table GCF)
-    ref_computed = None   # the plain FP64 scatter's grid (COMPUTE_GCF)
-    for name in args.variants.split(","):
+    for name in names:
         kw = dict(variants[name])
+        out32 = kw.pop("out32", False)
         if kw["strategy"] != "scatter":
             kw.update(tune)
+        computed = kw.get("gcf_mode", migrid.GCF_TABLE) == migrid.GCF_COMPAT
+        g_in = d_gcf_c if computed else d_gcf
         # the plain scatter runs ~1 s a call: fewer steps (the headline keeps K, W)
         plain = kw["strategy"] == "scatter"
         warmup, steps = (min(args.warmup, 1), min(args.steps, 3)) if plain else (args.warmup, args.steps)
+        if out32 and grid32 is None:
+            grid32 = torch.empty((N, N, 2), dtype=torch.float32, device="cuda")
+            grid_lo = torch.empty_like(grid32)
 
-        def step(kw=kw):
-            ctx.grid(grid, d_uv, d_vals, d_gcf, N, S, **kw)
+        def step(kw=kw, g_in=g_in, out32=out32):
+            if out32:
+                ctx.grid(grid32, d_uv, d_vals, g_in, N, S, grid_lo=grid_lo, **kw)
+            else:
+                ctx.grid(grid, d_uv, d_vals, g_in, N, S, **kw)
+        # the checked result: one untimed call into NaN-filled outputs, so a
+        # tile the kernels leave unwritten cannot keep an earlier answer
+        (grid32 if out32 else grid).fill_(float("nan"))
+        if out32:
+            grid_lo.fill_(float("nan"))
         try:
             step()
         except migrid.MigridError as e:
             print(f"# {name}: not available ({e})", file=sys.stderr, flush=True)
             continue
         torch.cuda.synchronize()
-        print(f"# {name}: timing {steps} steps", file=sys.stderr, flush=True)
-        wall, per, kern = time_variant(torch, ctx, step, warmup, steps)
-        g = grid.clone()
-        computed = kw.get("gcf_mode", migrid.GCF_TABLE) == migrid.GCF_COMPAT
-        if name == "fp64_plain":
-            ref_grid = g
-        if name == "fp64_plain_computed":
-            ref_computed = g
-        ref, ref_name = (ref_computed, "fp64_plain_computed") if computed else (ref_grid, "fp64_plain")
         err = None
-        if ref is not None and ref is not g:
+        ref = oracle.get("compat" if computed else "table")
+        if ref is not None:
+            g = grid32.double() + grid_lo.double() if out32 else grid
             d = torch.linalg.vector_norm(g - ref, dim=-1)
             m = torch.linalg.vector_norm(ref, dim=-1)
-            err = {"rel_max_vs_" + ref_name: float(d.max() / m.max()),
-                   "rel_rms_vs_" + ref_name: float(torch.sqrt((d * d).sum() / (m * m).sum()))}
+            err = {"rel_max_vs_oracle": float(d.max() / m.max()),
+                   "rel_rms_vs_oracle": float(torch.sqrt((d * d).sum() / (m * m).sum()))}
+            del g, d, m
+        print(f"# {name}: timing {steps} steps", file=sys.stderr, flush=True)
+        wall, per, kern = time_variant(torch, ctx, step, warmup, steps)
         ms = wall * 1e3 / steps
         results[name] = {"ms_per_step": ms, "mvis_per_s": npts / (ms * 1e-3) / 1e6,
                          "grid_point_updates_per_s": npts * S * S / (ms * 1e-3),
@@ -598,7 +678,6 @@ def main():
                          "kernel_ms_avg": float(np.mean(kern)) if kern else None,
                          "kernel_launches": len(kern), "steps": steps, "err": err}
         print(f"# {name}: {results[name]}", file=sys.stderr, flush=True)
-        del g
     if not results:
         print(json.dumps({"error": "no variant ran"}))
         sys.exit(1)
@@ -607,7 +686,7 @@ def main():
     R = results[head]
     flops = 8.0 * S * S * npts
     kms = R["kernel_ms_avg"]
-    fp32 = head in ("fp32_tiled", "bf16_tiled")
+    fp32 = head in ("fp32_tiled", "fp32_tiled_twosum", "bf16_tiled")
     peak = PEAK_FP32_TF if fp32 else PEAK_FP64_TF
     achieved = flops / (kms * 1e-3) / 1e12 if kms else None
     traffic = load_traffic().get(head)
@@ -625,7 +704,10 @@ def main():
         "dtype": "f32" if fp32 else "f64",
         "data": "synthetic",
         "config": {"workload": f"{N}x{N} complex128 grid, support {S}, oversampling 8, "
-                               f"{npts} radially clustered visibilities (Exp radius mean 800 px)",
+                               f"{npts} radially clustered visibilities (Exp radius mean 800 px)"
+                               if args.dist == "clustered" else
+                               f"{N}x{N} complex128 grid, support {S}, oversampling 8, "
+                               f"{npts} uniform visibilities",
                    "variant": head, "grid": N, "support": S, "npts": npts},
         "p50_ms": R["p50_ms"],
         "p99_ms": R["p99_ms"],
@@ -639,9 +721,37 @@ def main():
     if "fp64_plain" in results and head != "fp64_plain":
         out["speedup_vs_fp64_plain"] = R["mvis_per_s"] / results["fp64_plain"]["mvis_per_s"]
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(uv, vals, gcf, N, S, args.cpu_sample)
-        out["cpu_baseline_threads"] = cpu_baseline_threads(uv, vals, gcf, N, S, 4 * args.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(uv, vals, gcf, N, S, args.cpu_sample, workload)
+        out["cpu_baseline_threads"] = base_thr
     print(json.dumps(out), flush=True)
+
+
+def main_cpu(args):
+    """The CPU baselines in full: the reference's CPU-runnable case (2048^2,
+    S = 32, 1M uniform visibilities) through the single-threaded gridCPU
+    restatement and its OpenMP form, and the 8192^2 / S = 64 / 10M clustered
+    workload through the OpenMP form (single-threaded it would take most of
+    an hour)."""
+    O = oracle_mod()
+    lines = {}
+    uv, vals = O.gen_uniform(1_000_000, 64, 1984, seed=1541617)  # u, v in [64, 1984)
+    gcf = config_gcf(32)
+    t0 = time.perf_counter()
+    g1 = O.grid_zz(uv, vals, 2048, gcf, 32)
+    dt = time.perf_counter() - t0
+    lines["2048_s32_1m_single"] = {"value": 1.0 / dt, "unit": "Mvis/s", "cores": 1, "kind": "port",
+                                   "seconds": dt}
+    print(f"# cpu: 2048^2/S=32/1M single thread {dt:.1f} s", file=sys.stderr, flush=True)
+    gN, base = oracle_full(uv, vals, gcf, 2048, 32, "2048^2/S=32/1M uniform")
+    assert np.array_equal(g1, gN)
+    lines["2048_s32_1m_threads"] = base
+    del g1, gN
+    uv, vals = gen_clustered(args.npts, args.N, args.S)
+    _, base = oracle_full(uv, vals, config_gcf(args.S), args.N, args.S,
+                          f"{args.N}^2/S={args.S}/{args.npts} clustered")
+    lines[f"{args.N}_s{args.S}_{args.npts}_threads"] = base
+    print(json.dumps({"metric": "CPU gridCPU restatement, Mvis/s", "results": lines, **host_cpu()}),
+          flush=True)
 
 
 if __name__ == "__main__":

@@ -11,6 +11,8 @@
  *   oracle_grid_zz_rows      the same loop restricted to a band of output rows
  *                            (per-pixel summation order unchanged, so the
  *                            OpenMP row-band baseline is bit-identical)
+ *   oracle_grid_zz_omp,      row bands on OpenMP threads, rows outside and
+ *   oracle_grid_const_tap_omp  columns inside; per-pixel order unchanged
  *   oracle_init_gcf          init_gcf                     grid.cu:72-85
  *   oracle_ref_inputs        default input generator      grid.cu:444-452
  *   oracle_vis2ints          vis2ints                     grid_gpu.cu:352-365
@@ -120,10 +122,15 @@ void oracle_grid_zz_rows(double* out, const double* uv, const double* vals,
 }
 
 /* Rows [y_lo, y_hi) from the listed visibilities only (ascending n), each
- * visibility visiting just its rows inside the band. */
-static void grid_zz_list(double* out, const double* uv, const double* vals, const int64_t* list,
-                         int64_t count, int64_t N, const double* gcf, int64_t S, int64_t y_lo,
-                         int64_t y_hi)
+ * visibility visiting just its rows inside the band.  A visibility adds at
+ * most once to a pixel, so the order of its taps does not matter: rows are
+ * walked outside and columns inside (contiguous in each plane), and every
+ * pixel still receives its contributions in ascending n, bit-identical to
+ * the serial loop of grid.cu:115-133.  r2 == NULL: table taps; otherwise the
+ * constant tap (r2[n], i2[n]) of COMPUTE_GCF, as oracle_grid_const_tap. */
+__attribute__((optimize("tree-vectorize"))) static void grid_zz_list(double* out, const double* uv, const double* vals, const int64_t* list,
+                         int64_t count, int64_t N, const double* gcf, int64_t S, const float* r2c,
+                         const float* i2c, int64_t y_lo, int64_t y_hi)
 {
     double* re = out;
     double* im = out + N * N;
@@ -134,39 +141,54 @@ static void grid_zz_list(double* out, const double* uv, const double* vals, cons
         index_exact(uv[2 * n], &mx, &sx);
         index_exact(uv[2 * n + 1], &my, &sy);
         const int64_t k = (int64_t)OVERSAMPLE * sy + sx;
-        const double* g = gcf + 2 * k * S * S;
+        const double* g = gcf ? gcf + 2 * k * S * S : NULL;
         const double r1 = vals[2 * n], i1 = vals[2 * n + 1];
         int64_t b_hi = y_hi - 1 - my, b_lo = y_lo - my; /* rows of the band */
         if (b_hi > h) b_hi = h;
         if (b_lo < -h + 1) b_lo = -h + 1;
-        for (int64_t a = h; a > -h; a--) {
-            const int64_t x = mx + a;
-            if (x < 0 || x >= N) continue;
-            for (int64_t b = b_hi; b >= b_lo; b--) {
-                const int64_t y = my + b;
-                if (y < 0 || y >= N) continue;
-                const int64_t t = (b + h - 1) * S + (a + h - 1);
-                const double r2 = g[2 * t], i2 = g[2 * t + 1];
-                re[x + N * y] += r1 * r2 - i1 * i2;
-                im[x + N * y] += r1 * i2 + r2 * i1;
+        int64_t a_lo = -mx, a_hi = N - 1 - mx; /* columns on the grid */
+        if (a_lo < -h + 1) a_lo = -h + 1;
+        if (a_hi > h) a_hi = h;
+        if (r2c) {
+            const double pr = r1 * r2c[n] - i1 * i2c[n];
+            const double pi = r1 * i2c[n] + r2c[n] * i1;
+            for (int64_t b = b_lo; b <= b_hi; b++) {
+                double* rr = re + N * (my + b) + mx;
+                double* ri = im + N * (my + b) + mx;
+                for (int64_t a = a_lo; a <= a_hi; a++) {
+                    rr[a] += pr;
+                    ri[a] += pi;
+                }
+            }
+            continue;
+        }
+        for (int64_t b = b_lo; b <= b_hi; b++) {
+            double* rr = re + N * (my + b) + mx;
+            double* ri = im + N * (my + b) + mx;
+            const double* gt = g + 2 * ((b + h - 1) * S + (h - 1));
+            for (int64_t a = a_lo; a <= a_hi; a++) {
+                const double gr = gt[2 * a], gi = gt[2 * a + 1];
+                rr[a] += r1 * gr - i1 * gi;
+                ri[a] += r1 * gi + gr * i1;
             }
         }
     }
 }
 
-/* Multi-threaded baseline: OpenMP over bands of 16 rows.  One serial pass
- * lists each band's visibilities in ascending n; each thread then grids its
- * bands from their lists.  Every pixel still receives its contributions in
+/* Multi-threaded form: OpenMP over bands of 16 rows.  One serial pass lists
+ * each band's visibilities in ascending n; each thread then grids its bands
+ * from their lists.  Every pixel still receives its contributions in
  * ascending n, so the result is bit-identical to the serial loop.  Returns
  * the number of threads used (0 if the lists could not be allocated). */
-int oracle_grid_zz_omp(double* out, const double* uv, const double* vals, int64_t npts,
-                       int64_t N, const double* gcf, int64_t S, int nthreads)
+static int grid_omp(double* out, const double* uv, const double* vals, int64_t npts, int64_t N,
+                    const double* gcf, int64_t S, const float* r2c, const float* i2c, int nthreads)
 {
     memset(out, 0, sizeof(double) * 2 * N * N);
     const int64_t band = 16, nb = (N + band - 1) / band, h = S / 2;
     int64_t* start = (int64_t*)calloc((size_t)nb + 1, sizeof(int64_t));
     if (!start) return 0;
-    /* the band range of visibility n's rows [my - h + 1, my + h] on the grid */
+    /* the band range of visibility n's rows [my - h + 1, my + h] on the grid;
+     * empty when its window misses the grid's columns */
 #define OMP_BANDS(n, b0, b1)                                                     \
     int mx_, my_, sx_, sy_;                                                      \
     index_exact(uv[2 * (n)], &mx_, &sx_);                                        \
@@ -174,8 +196,10 @@ int oracle_grid_zz_omp(double* out, const double* uv, const double* vals, int64_
     int64_t r0_ = (int64_t)my_ - h + 1, r1_ = (int64_t)my_ + h;                  \
     if (r0_ < 0) r0_ = 0;                                                        \
     if (r1_ > N - 1) r1_ = N - 1;                                                \
+    const int cols_ = (int64_t)mx_ + h >= 0 && (int64_t)mx_ - h + 1 <= N - 1;    \
     const int64_t k_ = (int64_t)OVERSAMPLE * sy_ + sx_;                          \
-    const int64_t b0 = r0_ / band, b1 = (r0_ <= r1_ && k_ >= 0 && k_ < 64) ? r1_ / band : b0 - 1
+    const int64_t b0 = r0_ / band,                                               \
+                  b1 = (r0_ <= r1_ && cols_ && k_ >= 0 && k_ < 64) ? r1_ / band : b0 - 1
     for (int64_t n = 0; n < npts; n++) {
         OMP_BANDS(n, b0, b1);
         for (int64_t b = b0; b <= b1; b++) start[b + 1]++;
@@ -207,13 +231,31 @@ int oracle_grid_zz_omp(double* out, const double* uv, const double* vals, int64_
         /* interleaved bands balance clustered inputs */
         for (int64_t b = tid; b < nb; b += nt) {
             const int64_t y0 = b * band, y1 = y0 + band < N ? y0 + band : N;
-            grid_zz_list(out, uv, vals, list + start[b], start[b + 1] - start[b], N, gcf, S, y0, y1);
+            grid_zz_list(out, uv, vals, list + start[b], start[b + 1] - start[b], N, gcf, S, r2c,
+                         i2c, y0, y1);
         }
     }
     free(start);
     free(list);
     free(fill);
     return used;
+}
+
+/* gridCPU (grid.cu:87-139) on nthreads threads, bit-identical to
+ * oracle_grid_zz with exact index math and a zeroed grid. */
+int oracle_grid_zz_omp(double* out, const double* uv, const double* vals, int64_t npts,
+                       int64_t N, const double* gcf, int64_t S, int nthreads)
+{
+    return grid_omp(out, uv, vals, npts, N, gcf, S, NULL, NULL, nthreads);
+}
+
+/* oracle_grid_const_tap (the COMPUTE_GCF form) on nthreads threads,
+ * bit-identical to the serial loop. */
+int oracle_grid_const_tap_omp(double* out, const double* uv, const double* vals, int64_t npts,
+                              int64_t N, int64_t S, const float* r2, const float* i2,
+                              int nthreads)
+{
+    return grid_omp(out, uv, vals, npts, N, NULL, S, r2, i2, nthreads);
 }
 
 /* PRECISION=float, OUTPRECISION=double: float inputs and float products,

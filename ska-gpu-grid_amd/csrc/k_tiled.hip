@@ -293,7 +293,12 @@ __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs,
         if (det && n_items > 1) {  // list the split tile for the ordered reduction
             const int rb = atomicAdd(&qs->nreduce, 1);
             const int pb = atomicAdd(&qs->npartial, n_items);
+            // the plan's capacities bound every tile's need (tiled_plan), so
+            // this always fits; were it not to, the tile would flush with
+            // atomics (correct, not bitwise reproducible), counted in
+            // qs->overflow
             const bool fits = rb < jobs[job].rcap && pb + n_items <= jobs[job].pcap;
+            if (!fits) atomicAdd(&qs->overflow, 1);
             if (rb < jobs[job].rcap) {
                 ReduceItem ri;
                 ri.job = job;
@@ -1306,21 +1311,30 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     // the tap kernel's items: smaller by default, so dense tiles spread wider
     p.target_t = o.vis_per_wave > 0 ? o.vis_per_wave : 2048;
     // gather / window: split tiles are summed by an ordered reduction, not
-    // atomics.  A tile of c > target visibilities has at most
-    // ceil(c / target) < 2c / target items, so 2E / target + 1 partial tiles
-    // and E / target + 1 split tiles bound what the work-item pass can list.
+    // atomics.  Tile tx lists bins tx*T/H .. (tx*T + T + S - 2)/H (T is a
+    // multiple of H), so per axis a visibility's bin is listed by at most
+    // floor((T + S - 2) / T) + 1 tiles, and by m = that squared in all: the
+    // per-tile counts sum to at most m E.  A split tile holds c > target of
+    // them and makes at most ceil(c / target) < 2c / target items, so
+    // 2 m E / target + 1 partial tiles and m E / target + 1 split tiles bound
+    // what the work-item pass lists.  When those slots would pass 4 GiB the
+    // work items grow until they fit.
     p.det = o.strategy != MIGRID_TILED && !p.shared;
     if (p.det) {
-        const int64_t units = (p.E + p.target - 1) / p.target;
+        int64_t m = 1;
+        for (const TiledJob& t : p.jobs) {
+            const int64_t per_axis = (t.g.T + t.g.S - 2) / t.g.T + 1;
+            m = std::max<int64_t>(m, per_axis * per_axis);
+        }
         const size_t tsz = (p.key.stage == kStageF64 || p.key.acc64) ? 8 : 4;  // accumulator
-        // at most 4 GiB of partial tiles; a split tile that finds no room
-        // (only with tiny work items) flushes atomically instead
         const int64_t slot = (int64_t)p.key.T * p.key.T * 2 * (int64_t)tsz;
-        const int64_t max_slots = std::max<int64_t>(64, ((int64_t)4 << 30) / slot);
-        p.partial_cap = std::min<int64_t>(2 * units + 1, max_slots);
-        p.reduce_cap = std::min<int64_t>(units + 1, p.partial_cap);
+        const int64_t max_slots = ((int64_t)4 << 30) / slot;
+        if (2 * m * p.E / p.target + 1 > max_slots)
+            p.target = (2 * m * p.E + max_slots - 2) / (max_slots - 1);
+        p.partial_cap = 2 * m * p.E / p.target + 1;
+        p.reduce_cap = m * p.E / p.target + 1;
         p.off_partials = off;
-        off += align256((size_t)p.partial_cap * p.key.T * p.key.T * 2 * tsz);
+        off += align256((size_t)p.partial_cap * slot);
         p.off_reduce = off;
         off += align256((size_t)p.reduce_cap * sizeof(ReduceItem));
     }

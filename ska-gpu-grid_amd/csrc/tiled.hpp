@@ -40,7 +40,8 @@ struct QueueState {
     int32_t head;
     int32_t npartial;  // partial-tile slots taken
     int32_t nreduce;   // split tiles listed for the ordered reduction
-    int32_t pad[12];
+    int32_t overflow;  // split tiles that found no partial slot (none by the plan's bound)
+    int32_t pad[11];
 };
 
 // A split tile of the ordered-reduction mode: its n partial tiles are slots

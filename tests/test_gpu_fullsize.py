@@ -1,15 +1,27 @@
-"""Parity at the benchmark's full sizes.
+"""Parity at the benchmark's full sizes, against the oracle's gridCPU.
 
-2048^2 grid, S = 32, 1M uniform visibilities (the reference's CPU-runnable
-case): every strategy against the oracle's gridCPU (row-band OpenMP form,
-bit-identical to the serial loop), FP64 within 1e-12, FP32 tiles within 1e-5.
+The oracle is gridCPU restated in C (grid.cu:87-139), in its row-band
+OpenMP form, which is bit-identical to the serial loop.  On the GPU host's
+cores it grids 10M visibilities at S = 64 in tens of seconds, so every
+check here is against it, at the workload's real size:
 
-8192^2 grid, S = 64, 10M radially clustered visibilities: too big for the
-oracle in a test, so size-independent properties: every FP64 strategy agrees
-with the plain FP64 scatter within 1e-12 and FP32 tiles within 1e-5; the
-coverage map (DEBUG1: +1 per tap) is bit-identical between the scatter and
-the tiled kernel and sums to exactly npts * S^2; gridding is linear
-(values x 2 -> grid x 2, exact up to summation order).
+* 2048^2 grid, S = 32, 1M uniform visibilities (the reference's
+  CPU-runnable case): scatter, tiled, gather and window (planar, its
+  reference layout) within 1e-12; FP32 tiles within 1e-5; BF16 tables
+  reported within 2e-2.
+* 8192^2 grid, S = 64, 10M radially clustered visibilities: the same
+  strategies within 1e-12, FP32 tiles into the FP64 grid and into an FP32
+  grid with a two-sum compensation grid within 1e-5, BF16 within 2e-2, and
+  COMPUTE_GCF (grid_gpu.cu:103-108,137-147) in both phase forms: the
+  scatter form for scatter and tiled, the int2 fixed-point form for gather
+  and window (grid_gpu.cu:456-459,600-603), each against the oracle's
+  constant-tap gridder.
+* Gather and window are bitwise reproducible: two runs are equal.
+* The coverage map (DEBUG1: +1 per tap) is bit-identical between the
+  scatter and the tiled kernel and sums to exactly npts * S^2.
+
+Every output buffer is filled with NaN before each call, so a tile that a
+kernel fails to write shows up as NaN instead of as a previous result.
 """
 import os
 
@@ -21,6 +33,8 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
+NAN = float("nan")
+
 
 def torch():
     import torch as t
@@ -31,15 +45,70 @@ def dev(a):
     return torch().from_numpy(np.ascontiguousarray(a)).cuda()
 
 
-def rel_max(a, b):
+def interleaved(g, lo=None):
+    """(N, N, 2) float64 view of a grid: interleaved as is, planar (2, N, N)
+    moved, FP32 grids widened and their compensation grid added."""
     t = torch()
-    d = t.linalg.vector_norm((a - b).reshape(-1, 2) if a.shape[-1] == 2 else (a - b).reshape(2, -1).T, dim=-1)
-    m = t.linalg.vector_norm(b.reshape(-1, 2) if b.shape[-1] == 2 else b.reshape(2, -1).T, dim=-1)
-    return float(d.max() / m.max())
+    g = g.double()
+    if lo is not None:
+        g = g + lo.double()
+    if g.shape[0] == 2 and g.shape[-1] != 2:
+        g = g.permute(1, 2, 0)
+    return g
+
+
+def errors(got, ref):
+    """(rel_max, rel_rms) over complex moduli, as O.rel_errors; NaN anywhere
+    in got makes both NaN."""
+    t = torch()
+    d = t.linalg.vector_norm(got - ref, dim=-1)
+    m = t.linalg.vector_norm(ref, dim=-1)
+    return float(d.max() / m.max()), float(t.sqrt((d * d).sum() / (m * m).sum()))
 
 
 def threads():
     return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def planar_ref(ref):
+    return torch().from_numpy(np.ascontiguousarray(np.moveaxis(ref, 0, -1))).cuda()
+
+
+def compat_table(gcf, S, T, w):
+    """A copy of gcf whose element S(S-1)/2-1 holds (T, w), the two numbers
+    COMPUTE_GCF reads from the host table (grid_gpu.cu:103-108)."""
+    g = np.array(gcf, np.float64, copy=True)
+    g[S * (S - 1) // 2 - 1] = (T, w)
+    return g
+
+
+def compat_tol(uv, T, w, fixed):
+    """The phase is a float expression: a 1-ulp difference in sqrtf or in the
+    sin / cos argument reduction moves the tap value by about |phase| 2^-24."""
+    f32 = np.float32
+    p1 = f32(2 * 3.1415926 * w)
+    p2 = f32(float(p1) * T)
+    m = np.floor(uv)
+    sub = np.floor(8 * (uv - m))
+    base = m * 8 + sub if fixed else uv
+    d = (m - base + sub / 8.0).astype(f32)
+    phase = np.abs(p1 - p2 * np.sqrt((d * d).sum(1)))
+    return max(1e-6, 8 * float(phase.max()) * 2.0 ** -24)
+
+
+STRATS_64 = [("scatter", {}), ("tiled", {}), ("gather", {}), ("window", {}),
+             ("window", dict(layout=migrid.LAYOUT_INTERLEAVED))]
+
+
+def grid_fresh(ctx, N, S, uv, vals, gcf, strategy, **kw):
+    """A fresh NaN-filled output of the strategy's layout, gridded."""
+    t = torch()
+    layout = kw.get("layout", migrid.LAYOUT_DEFAULT)
+    planar = layout == migrid.LAYOUT_PLANAR or (layout == migrid.LAYOUT_DEFAULT and strategy == "window")
+    g = t.full((2, N, N) if planar else (N, N, 2), NAN, dtype=t.float64, device="cuda")
+    ctx.grid(g, uv, vals, gcf, N, S, strategy=strategy, **kw)
+    t.cuda.synchronize()
+    return g
 
 
 def test_2048_s32_1m_matches_gridcpu():
@@ -49,53 +118,126 @@ def test_2048_s32_1m_matches_gridcpu():
     uv, vals = O.gen_uniform(n, 64, 1984, seed=1541617)
     gcf = O.config_gcf(S)
     ref, _ = O.grid_zz_omp(uv, vals, N, gcf, S, threads())
-    ref_t = t.from_numpy(np.ascontiguousarray(np.moveaxis(ref, 0, -1))).cuda()
+    ref_t = planar_ref(ref)
     d_uv, d_vals, d_gcf = dev(uv), dev(vals), dev(gcf)
-    grid = t.empty((N, N, 2), dtype=t.float64, device="cuda")
     ctx = migrid.Context()
     for strategy, kw, tol in (("scatter", {}, 1e-12), ("tiled", {}, 1e-12), ("gather", {}, 1e-12),
+                              ("window", {}, 1e-12),
                               ("tiled", dict(arith=migrid.ARITH_F32), 1e-5),
                               ("tiled", dict(arith=migrid.ARITH_F32_BF16GCF), 2e-2)):
-        ctx.grid(grid, d_uv, d_vals, d_gcf, N, S, strategy=strategy, layout=migrid.LAYOUT_INTERLEAVED,
-                 **kw)
-        t.cuda.synchronize()
-        e = rel_max(grid, ref_t)
-        print(f"  {strategy} {kw}: rel_max {e:.3e}", flush=True)
-        assert e < tol, (strategy, kw, e)
+        g = grid_fresh(ctx, N, S, d_uv, d_vals, d_gcf, strategy, **kw)
+        e = errors(interleaved(g), ref_t)
+        print(f"  {strategy} {kw}: rel_max {e[0]:.3e} rel_rms {e[1]:.3e}", flush=True)
+        assert e[0] < tol, (strategy, kw, e)
+        del g
 
 
 @pytest.fixture(scope="module")
 def big():
-    t = torch()
     N, S, n = 8192, 64, 10_000_000
+    print(f"[8192^2 S=64 10M] inputs", flush=True)
     uv, vals = O.gen_clustered(n, N, S, mean_r=800.0, seed=1541618)
-    d = dict(N=N, S=S, n=n, uv=dev(uv), vals=dev(vals), gcf=dev(O.config_gcf(S)))
-    del uv, vals
-    ctx = migrid.Context()
-    ref = t.empty((N, N, 2), dtype=t.float64, device="cuda")
-    ctx.grid(ref, d["uv"], d["vals"], d["gcf"], N, S, strategy="scatter")
-    t.cuda.synchronize()
-    d.update(ctx=ctx, ref=ref)
+    gcf = O.config_gcf(S)
+    d = dict(N=N, S=S, n=n, uv_h=uv, vals_h=vals, gcf_h=gcf, uv=dev(uv), vals=dev(vals),
+             gcf=dev(gcf), ctx=migrid.Context(), refs={})
     yield d
-    del d
-    t.cuda.empty_cache()
+    d.clear()
+    torch().cuda.empty_cache()
 
 
-@pytest.mark.parametrize("strategy,kw,tol", [
-    ("tiled", {}, 1e-12), ("gather", {}, 1e-12),
-    ("window", dict(layout=migrid.LAYOUT_INTERLEAVED), 1e-12),
-    ("tiled", dict(arith=migrid.ARITH_F32), 1e-5),
-])
-def test_8192_s64_10m_strategies_agree(big, strategy, kw, tol):
+def oracle_grid(big, name):
+    """The oracle's grid for one GCF form, computed once per module on the
+    host's cores and kept on the device (interleaved)."""
+    if name in big["refs"]:
+        return big["refs"][name]
+    N, S = big["N"], big["S"]
+    uv, vals = big["uv_h"], big["vals_h"]
+    print(f"  oracle ({name}) on {threads()} threads", flush=True)
+    if name == "table":
+        ref, _ = O.grid_zz_omp(uv, vals, N, big["gcf_h"], S, threads())
+    else:
+        T, w, fixed = COMPAT[name]
+        r2, i2 = O.compute_gcf_phase(uv, T, w, fixed_point=fixed)
+        ref, _ = O.grid_const_tap_omp(uv, vals, N, S, r2, i2, threads())
+    big["refs"][name] = planar_ref(ref)
+    return big["refs"][name]
+
+
+@pytest.mark.parametrize("strategy,kw", STRATS_64)
+def test_8192_s64_10m_fp64_matches_gridcpu(big, strategy, kw):
+    N, S = big["N"], big["S"]
+    ref = oracle_grid(big, "table")
+    print(f"[8192^2 S=64 10M] {strategy} {kw}", flush=True)
+    g = grid_fresh(big["ctx"], N, S, big["uv"], big["vals"], big["gcf"], strategy, **kw)
+    e = errors(interleaved(g), ref)
+    print(f"  rel_max {e[0]:.3e} rel_rms {e[1]:.3e} vs gridCPU", flush=True)
+    assert e[0] < 1e-12
+
+
+@pytest.mark.parametrize("name,arith,tol", [("fp32", migrid.ARITH_F32, 1e-5),
+                                            ("bf16", migrid.ARITH_F32_BF16GCF, 2e-2)])
+def test_8192_s64_10m_mixed_precision(big, name, arith, tol):
+    """FP32 products and tile sums flushed into the FP64 grid, and into an
+    FP32 grid with its two-sum compensation grid; max and RMS error against
+    the FP64 oracle."""
     t = torch()
     N, S = big["N"], big["S"]
-    print(f"[8192^2 S=64 10M] {strategy} {kw}", flush=True)
-    g = t.empty((N, N, 2), dtype=t.float64, device="cuda")
-    big["ctx"].grid(g, big["uv"], big["vals"], big["gcf"], N, S, strategy=strategy, **kw)
+    ref = oracle_grid(big, "table")
+    g = grid_fresh(big["ctx"], N, S, big["uv"], big["vals"], big["gcf"], "tiled", arith=arith)
+    e64 = errors(interleaved(g), ref)
+    del g
+    g32 = t.full((N, N, 2), NAN, dtype=t.float32, device="cuda")
+    lo = t.full_like(g32, NAN)
+    big["ctx"].grid(g32, big["uv"], big["vals"], big["gcf"], N, S, grid_lo=lo, strategy="tiled",
+                    arith=arith)
     t.cuda.synchronize()
-    e = rel_max(g, big["ref"])
-    print(f"  rel_max vs scatter {e:.3e}", flush=True)
-    assert e < tol
+    e32 = errors(interleaved(g32, lo), ref)
+    e32_plain = errors(interleaved(g32), ref)
+    print(f"[8192^2 S=64 10M] {name} tiles: FP64 grid rel_max {e64[0]:.3e} rms {e64[1]:.3e}; "
+          f"FP32 + two-sum rel_max {e32[0]:.3e} rms {e32[1]:.3e} "
+          f"(FP32 grid alone {e32_plain[0]:.3e})", flush=True)
+    assert e64[0] < tol and e32[0] < tol
+    assert not t.isnan(lo).any()
+
+
+# COMPUTE_GCF tables: (T, w, fixed point).  The scatter form's offsets are
+# below an eighth of a pixel, so T is large enough to move the phase; the
+# fixed-point form's offsets are about 7 * 8 * main (tens of thousands at
+# 8192^2), so T is small to keep the phase within a few radians.
+COMPAT = {"compat_scatter": (2.0, 0.31, False), "compat_fixed": (1e-4, 0.31, True)}
+
+
+@pytest.mark.parametrize("name,strategies", [("compat_scatter", ("scatter", "tiled")),
+                                             ("compat_fixed", ("gather", "window"))])
+def test_8192_s64_10m_compute_gcf(big, name, strategies):
+    N, S = big["N"], big["S"]
+    T, w, fixed = COMPAT[name]
+    ref = oracle_grid(big, name)
+    tol = compat_tol(big["uv_h"], T, w, fixed)
+    gcf = dev(compat_table(big["gcf_h"], S, T, w))
+    got = []
+    for strategy in strategies:
+        g = grid_fresh(big["ctx"], N, S, big["uv"], big["vals"], gcf, strategy,
+                       gcf_mode=migrid.GCF_COMPAT)
+        got.append(interleaved(g))
+        e = errors(got[-1], ref)
+        print(f"[8192^2 S=64 10M] COMPUTE_GCF {strategy}: rel_max {e[0]:.3e} rms {e[1]:.3e} "
+              f"(tolerance {tol:.1e} from the float phase)", flush=True)
+        assert e[0] < tol
+    # both strategies evaluate the same float tap value on the device
+    e = errors(got[1], got[0])
+    print(f"  {strategies[1]} vs {strategies[0]}: rel_max {e[0]:.3e}", flush=True)
+    assert e[0] < 1e-12
+
+
+@pytest.mark.parametrize("strategy", ["gather", "window"])
+def test_8192_s64_10m_reproducible(big, strategy):
+    """Split tiles are summed in a fixed order: two runs agree bit for bit."""
+    t = torch()
+    N, S = big["N"], big["S"]
+    a = grid_fresh(big["ctx"], N, S, big["uv"], big["vals"], big["gcf"], strategy)
+    b = grid_fresh(big["ctx"], N, S, big["uv"], big["vals"], big["gcf"], strategy)
+    assert t.equal(a, b)
 
 
 def test_8192_s64_10m_coverage_bit_exact_and_linear(big):
@@ -103,15 +245,11 @@ def test_8192_s64_10m_coverage_bit_exact_and_linear(big):
     N, S, n = big["N"], big["S"], big["n"]
     ctx = big["ctx"]
     print("[8192^2 S=64 10M] coverage + linearity", flush=True)
-    a = t.empty((N, N, 2), dtype=t.float64, device="cuda")
-    b = t.empty_like(a)
-    ctx.grid(a, big["uv"], big["vals"], big["gcf"], N, S, strategy="scatter", debug_coverage=True)
-    ctx.grid(b, big["uv"], big["vals"], big["gcf"], N, S, strategy="tiled", debug_coverage=True)
-    t.cuda.synchronize()
+    a = grid_fresh(ctx, N, S, big["uv"], big["vals"], big["gcf"], "scatter", debug_coverage=True)
+    b = grid_fresh(ctx, N, S, big["uv"], big["vals"], big["gcf"], "tiled", debug_coverage=True)
     assert t.equal(a, b)
     assert float(a[..., 0].sum()) == float(n) * S * S
-    del a
-    v2 = big["vals"] * 2.0
-    ctx.grid(b, big["uv"], v2, big["gcf"], N, S, strategy="tiled")
-    t.cuda.synchronize()
-    assert rel_max(b, 2.0 * big["ref"]) < 1e-12
+    del a, b
+    ref = oracle_grid(big, "table")
+    b = grid_fresh(ctx, N, S, big["uv"], big["vals"] * 2.0, big["gcf"], "tiled")
+    assert errors(interleaved(b), 2.0 * ref)[0] < 1e-12

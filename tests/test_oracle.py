@@ -119,12 +119,16 @@ def test_row_bands_and_openmp_are_bit_identical():
     part = O.grid_zz_rows(uv, vals, N, gcf, S, 0, 100)
     O.grid_zz_rows(uv, vals, N, gcf, S, 100, N, out=part)
     np.testing.assert_array_equal(part, ref)
-    # windows crossing every edge of a grid that is not a multiple of the band
-    for S, N in ((16, 77), (64, 50), (2, 9)):
+    # windows crossing every edge of a grid that is not a multiple of the band,
+    # for both GCF tables and the constant-tap (COMPUTE_GCF) form
+    for S, N in ((16, 77), (64, 50), (2, 9), (64, 300)):
         uv, vals = O.gen_uniform(2000, -S, N + S, seed=S + N)
-        gcf = O.config_gcf(S)
-        got, _ = O.grid_zz_omp(uv, vals, N, gcf, S, 3)
-        np.testing.assert_array_equal(got, O.grid_zz(uv, vals, N, gcf, S))
+        for gcf in (O.config_gcf(S), O.init_gcf(S)):
+            got, _ = O.grid_zz_omp(uv, vals, N, gcf, S, 3)
+            np.testing.assert_array_equal(got, O.grid_zz(uv, vals, N, gcf, S))
+        r2, i2 = O.compute_gcf_phase(uv, 0.3, 0.2)
+        got, _ = O.grid_const_tap_omp(uv, vals, N, S, r2, i2, 3)
+        np.testing.assert_array_equal(got, O.grid_const_tap(uv, vals, N, S, r2, i2))
 
 
 def test_floorf_emulation_K4():
@@ -257,20 +261,93 @@ def test_window_pixel_brute_force():
                 assert p == (cands[0] if cands[0] < N else -1)
 
 
-def test_compute_gcf_phase_float_semantics():
-    uv, vals = O.gen_uniform(64, 0, 128, seed=12)
+def _phase_p(w, T):
+    """p1 = (float)(2 * 3.1415926 * w) and p2 = (float)(p1 * T): the host-side
+    constants of COMPUTE_GCF (grid_gpu.cu:103-108)."""
+    p1 = float(np.float32(2 * 3.1415926 * w))
+    p2 = float(np.float32(p1 * T))
+    return p1, p2
+
+
+def _assert_tap(r2, i2, phi, err_phi):
+    """(r2, i2) = (sin, cos) of an exact phase phi, to within the float
+    rounding of the phase (err_phi) and of sinf / cosf (2 ulp of 1)."""
+    tol = err_phi + 2.0 ** -22
+    np.testing.assert_allclose(r2.astype(np.float64), np.sin(phi), rtol=0, atol=tol)
+    np.testing.assert_allclose(i2.astype(np.float64), np.cos(phi), rtol=0, atol=tol)
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_compute_gcf_phase_w0_and_T0(form):
+    """w = 0 makes p1 = p2 = 0, so every tap is (sin 0, cos 0) = (0, 1) exactly;
+    T = 0 makes p2 = 0, so the phase is p1 for every visibility whatever its
+    offset: (r2, i2) = (sinf(p1), cosf(p1)) (sin first, grid_gpu.cu:146-147)."""
+    uv, _ = O.gen_uniform(200, 0, 500, seed=31 + form)
+    r2, i2 = O.compute_gcf_phase(uv, 0.7, 0.0, fixed_point=bool(form))
+    np.testing.assert_array_equal(r2, np.zeros(200, np.float32))
+    np.testing.assert_array_equal(i2, np.ones(200, np.float32))
+    w = 0.31
+    p1, _ = _phase_p(w, 0.0)
+    r2, i2 = O.compute_gcf_phase(uv, 0.0, w, fixed_point=bool(form))
+    assert (r2 == r2[0]).all() and (i2 == i2[0]).all()
+    _assert_tap(r2, i2, np.full(200, p1), 0.0)
+    assert r2[0] != i2[0]  # (sin, cos), not (cos, sin): p1 is not pi/4
+
+
+def test_compute_gcf_phase_form0_offsets():
+    """Scatter form: dx = (float)(main - u + sub/8) is minus the position's
+    remainder below its eighth.  uv on exact eighths: dx = dy = 0, phase = p1
+    for any T.  An offset of 1/16 in u alone: |d| = 1/16, phase = p1 - p2/16."""
     T, w = 0.7, 0.31
+    p1, p2 = _phase_p(w, T)
+    m = np.array([3, 17, 250, 1000])
+    s = np.array([0, 3, 5, 7])
+    uv = np.stack([m + s / 8.0, m[::-1] + s[::-1] / 8.0], 1)
     r2, i2 = O.compute_gcf_phase(uv, T, w)
-    f32 = np.float32
-    p1 = f32(2 * 3.1415926 * w)
-    p2 = f32(float(p1) * T)
-    m = np.floor(uv)
-    sub = np.floor(8 * (uv - m))
-    xs = (m[:, 0] - uv[:, 0] + sub[:, 0] / 8.0).astype(f32)
-    ys = (m[:, 1] - uv[:, 1] + sub[:, 1] / 8.0).astype(f32)
-    ph = p1 - p2 * np.sqrt(xs * xs + ys * ys)
-    np.testing.assert_allclose(r2, np.sin(ph.astype(np.float64)), atol=2e-7)
-    np.testing.assert_allclose(i2, np.cos(ph.astype(np.float64)), atol=2e-7)
+    _assert_tap(r2, i2, np.full(4, p1), 0.0)
+    uv2 = uv.copy()
+    uv2[:, 0] += 1.0 / 16.0
+    r2, i2 = O.compute_gcf_phase(uv2, T, w)
+    phi = p1 - p2 / 16.0
+    _assert_tap(r2, i2, np.full(4, phi), 2.0 ** -22 * 4)
+    # the fixed-point form ignores the 1/16 (the fixed point keeps eighths)
+    r2f, i2f = O.compute_gcf_phase(uv2, T, w, fixed_point=True)
+    r2e, i2e = O.compute_gcf_phase(uv, T, w, fixed_point=True)
+    np.testing.assert_array_equal(r2f, r2e)
+    np.testing.assert_array_equal(i2f, i2e)
+
+
+def test_compute_gcf_phase_form1_fixed_point_distance():
+    """Gather / moving-window form: u is replaced by the int2 fixed point
+    X = 8 main + sub (grid_gpu.cu:456-459, :600-603), so dx = main - X + sub/8
+    = -7 X / 8 exactly and |d| = 7/8 sqrt(X^2 + Y^2).  main = sub = 0 gives
+    phase p1; (X, Y) on 3-4-5 triangles give exact distances."""
+    T, w = 0.7, 0.31
+    p1, p2 = _phase_p(w, T)
+    # (u, v) -> (X, Y): exact eighths so main and sub are as written
+    cases = [((0.0, 0.0), 0.0),                 # X = Y = 0
+             ((0.05, 0.1), 0.0),                # still main = sub = 0 (below 1/8)
+             ((3 / 8.0, 4 / 8.0), 35 / 8.0),    # X = 3, Y = 4
+             ((3.0, 4.0), 35.0),                # X = 24, Y = 32
+             ((5.125, 0.0), 7 * 41 / 8.0),      # X = 41, Y = 0
+             ((7.5, 10.0), 7 * 100 / 8.0)]      # X = 60, Y = 80: 7/8 * 100
+    uv = np.array([c[0] for c in cases])
+    dist = np.array([c[1] for c in cases])
+    r2, i2 = O.compute_gcf_phase(uv, T, w, fixed_point=True)
+    phi = p1 - p2 * dist
+    # float rounding of p2 * |d| and of the difference
+    err = 2.0 ** -23 * (np.abs(p2 * dist) + np.abs(phi)) * 2
+    for n in range(len(cases)):
+        _assert_tap(r2[n:n + 1], i2[n:n + 1], phi[n:n + 1], float(err[n]))
+    # the scatter form sees the exact eighths at distance 0 (all but (0.05, 0.1))
+    ex = np.array([0, 2, 3, 4, 5])
+    r0, i0 = O.compute_gcf_phase(uv[ex], T, w)
+    _assert_tap(r0, i0, np.full(len(ex), p1), 0.0)
+
+
+def test_compute_gcf_phase_grid_is_value_times_constant_tap():
+    uv, vals = O.gen_uniform(64, 0, 128, seed=12)
+    r2, i2 = O.compute_gcf_phase(uv, 0.7, 0.31)
     S, N = 8, 128
     g = O.grid_const_tap(uv, vals, N, S, r2, i2)
     # independent: constant tap value = impulse with an all-ones table times value
@@ -279,6 +356,9 @@ def test_compute_gcf_phase_float_semantics():
     v = (vals[:, 0] + 1j * vals[:, 1]) * (r2.astype(np.float64) + 1j * i2.astype(np.float64))
     ref = numpy_grid(uv, np.stack([v.real, v.imag], 1), N, tbl, S)
     assert O.rel_errors(g, ref)[0] < 1e-14
+    got, used = O.grid_const_tap_omp(uv, vals, N, S, r2, i2, 3)
+    assert used >= 1
+    np.testing.assert_array_equal(got, g)
 
 
 def test_float_pairs_close_to_double():
