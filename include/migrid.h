@@ -177,6 +177,20 @@ int migrid_ctx_destroy(migrid_ctx* ctx);
 int migrid_ctx_grid(migrid_ctx* ctx, const migrid_job* job, const migrid_opts* opts,
                     void* stream);
 
+/* Grid one job like migrid_ctx_grid, allocating nothing.  No counterpart
+ * in the reference: this is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The tiled strategies (TILED, GATHER, WINDOW) take their device
+ * scratch -- bin sort, records, work queue, partial tiles -- from one static
+ * device array of MIGRID_NOALLOC_SCRATCH_BYTES per GPU, which the library
+ * holds until the process ends; a job that needs more fails with
+ * MIGRID_ENOTSUP before any launch (grid it through a context instead).
+ * SCATTER needs no scratch.  All calls on one GPU share that array: calls
+ * that overlap on one GPU race, so callers serialise them.  No context, no
+ * profiling events.  Asynchronous on stream (a hipStream_t, or NULL). */
+#define MIGRID_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int migrid_grid_noalloc(const migrid_job* job, const migrid_opts* opts, void* stream);
+
 /* Bin sort only (device pointers): stable order of visibility indices by
  * key (k, bin_y, bin_x) with k = 8*sub_y + sub_x, bins of `bin` pixels over
  * q = floor(u) + S/2 in [0, N+S-2]; perm[0..count) and CSR offsets

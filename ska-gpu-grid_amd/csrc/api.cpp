@@ -108,11 +108,9 @@ int launch_tiled(migrid_ctx* ctx, const GridArgs& a, const migrid_opts& o, hipSt
 
 static size_t elem_bytes(int t) { return t == MIGRID_F64 ? 8 : 4; }
 
-// Grid one job whose buffers are already on the device.
-static int grid_device(migrid_ctx* ctx, const migrid_job* job, const migrid_opts* opts_in,
-                       hipStream_t s)
+int device_job_args(const migrid_job* job, const migrid_opts* opts_in, GridArgs& a,
+                    migrid_opts& o)
 {
-    migrid_opts o;
     if (opts_in)
         o = *opts_in;
     else
@@ -138,7 +136,6 @@ static int grid_device(migrid_ctx* ctx, const migrid_job* job, const migrid_opts
     if (o.gcf_mode < MIGRID_GCF_TABLE || o.gcf_mode > MIGRID_GCF_ANALYTIC)
         return fail(MIGRID_EINVAL, "gcf_mode=%d", o.gcf_mode);
 
-    GridArgs a;
     a.uv = job->uv;
     a.vals = job->vals;
     a.gcf = job->gcf;
@@ -156,6 +153,16 @@ static int grid_device(migrid_ctx* ctx, const migrid_job* job, const migrid_opts
     a.sigma = o.gcf_sigma > 0 ? o.gcf_sigma : a.S / 6.0;
     a.beta = o.gcf_beta;
 
+    return MIGRID_OK;
+}
+
+// Grid one job whose buffers are already on the device.
+static int grid_device(migrid_ctx* ctx, const migrid_job* job, const migrid_opts* opts_in,
+                       hipStream_t s)
+{
+    GridArgs a;
+    migrid_opts o;
+    MG_TRY(device_job_args(job, opts_in, a, o));
     if (o.strategy == MIGRID_SCATTER) {
         if (!o.accumulate) {
             size_t gbytes = (size_t)a.N * a.N * 2 * elem_bytes(a.out_type);

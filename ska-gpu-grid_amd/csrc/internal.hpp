@@ -61,6 +61,15 @@ int validate_shape(int64_t npts, int64_t N, int64_t S);
 
 // ---- launchers (each returns MIGRID_OK or a recorded error) ------------
 int launch_scatter(const GridArgs& a, hipStream_t s);
+// The tiled strategies of one job on caller-provided device scratch of cap
+// bytes (bin sort, records, work queue); fails with MIGRID_ENOTSUP, before
+// any launch, when the job needs more.
+int launch_tiled_scratch(const GridArgs& a, const migrid_opts& o, void* scratch, size_t cap,
+                         hipStream_t s);
+// Checks a device job and its options and fills the kernel-facing
+// arguments (the checks of migrid_ctx_grid).
+int device_job_args(const migrid_job* job, const migrid_opts* opts_in, GridArgs& a,
+                    migrid_opts& o);
 int launch_zero(void* p, size_t bytes, hipStream_t s);  // fill kernel, graph-safe
 int fill_u32(void* p, uint32_t v, int64_t n, hipStream_t s);
 

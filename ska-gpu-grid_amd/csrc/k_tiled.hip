@@ -1505,29 +1505,57 @@ int tiled_execute(const TiledPlan& p, char* arena, hipStream_t s)
     return MIGRID_OK;
 }
 
+// One job through the tiled strategies: the plan, then (unless there is
+// nothing to grid) its passes on `arena` (p.arena_bytes) and `bins`
+// (p.bin_bytes), which the caller provides.
+static int run_single(const TiledPlan& p, const GridArgs& a, char* arena, void* bins,
+                      Profiler* prof, hipStream_t s)
+{
+    std::vector<JobDesc> d;
+    tiled_descs(p, arena, d);
+    MG_TRY(tiled_write_descs(p, arena, d, s));
+    MG_TRY(tiled_prepare(p, arena, bins, s));
+    if (prof) MG_TRY(prof->begin(s));
+    MG_TRY(tiled_execute(p, arena, s));
+    if (prof) MG_TRY(prof->end(s));
+    return MIGRID_OK;
+}
+
+// Nothing to grid: a fresh grid is only cleared (fill kernels).
+static int clear_only(const TiledPlan& p, const GridArgs& a, hipStream_t s)
+{
+    if (!p.fresh) return MIGRID_OK;
+    const size_t gb = (size_t)a.N * a.N * 2 * (a.out_type == MIGRID_F64 ? 8 : 4);
+    MG_TRY(launch_zero(a.grid, gb, s));
+    if (a.grid_lo) MG_TRY(launch_zero(a.grid_lo, gb, s));
+    return MIGRID_OK;
+}
+
 int launch_tiled(migrid_ctx* ctx, const GridArgs& a, const migrid_opts& o, hipStream_t s)
 {
     if (!ctx) return fail(MIGRID_EINVAL, "tiled strategies need a context");
     TiledPlan p;
     MG_TRY(tiled_plan(&a, 1, o, p));
-    if (a.npts == 0) {  // nothing to grid: a fresh grid is only cleared
-        if (!p.fresh) return MIGRID_OK;
-        const size_t gb = (size_t)a.N * a.N * 2 * (a.out_type == MIGRID_F64 ? 8 : 4);
-        MG_TRY(launch_zero(a.grid, gb, s));
-        if (a.grid_lo) MG_TRY(launch_zero(a.grid_lo, gb, s));
-        return MIGRID_OK;
-    }
+    if (a.npts == 0) return clear_only(p, a, s);
     MG_TRY(ctx->bins.ensure(p.bin_bytes));
     MG_TRY(ctx->records.ensure(p.arena_bytes));
-    char* arena = (char*)ctx->records.p;
-    std::vector<JobDesc> d;
-    tiled_descs(p, arena, d);
-    MG_TRY(tiled_write_descs(p, arena, d, s));
-    MG_TRY(tiled_prepare(p, arena, ctx->bins.p, s));
-    MG_TRY(ctx->prof.begin(s));
-    MG_TRY(tiled_execute(p, arena, s));
-    MG_TRY(ctx->prof.end(s));
-    return MIGRID_OK;
+    return run_single(p, a, (char*)ctx->records.p, ctx->bins.p, &ctx->prof, s);
+}
+
+int launch_tiled_scratch(const GridArgs& a, const migrid_opts& o, void* scratch, size_t cap,
+                         hipStream_t s)
+{
+    TiledPlan p;
+    MG_TRY(tiled_plan(&a, 1, o, p));
+    if (a.npts == 0) return clear_only(p, a, s);
+    const size_t need = align256(p.arena_bytes) + p.bin_bytes;
+    if (need > cap)
+        return fail(MIGRID_ENOTSUP,
+                    "this job needs %zu bytes of device scratch, beyond the %zu bytes of the "
+                    "allocation-free entry point (grid it through a context instead)",
+                    need, cap);
+    char* arena = (char*)scratch;
+    return run_single(p, a, arena, arena + align256(p.arena_bytes), nullptr, s);
 }
 
 }  // namespace migrid

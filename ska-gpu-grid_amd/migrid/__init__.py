@@ -106,7 +106,7 @@ EXPORTS = [
     "migrid_server_wait", "migrid_server_read_grid", "migrid_server_drop_job",
     "migrid_server_get_stats", "migrid_server_destroy",
     "migrid_uvw_to_pixels", "migrid_ctx_grid_indexed", "migrid_read_uvw_text",
-    "migrid_ctx_degrid", "migrid_ctx_fft",
+    "migrid_ctx_degrid", "migrid_ctx_fft", "migrid_grid_noalloc",
 ]
 
 _lib = None
@@ -170,6 +170,7 @@ def lib():
     L.migrid_read_uvw_text.argtypes = [ctypes.c_char_p, i64, i64, vp, vp, ctypes.POINTER(i64)]
     L.migrid_ctx_degrid.argtypes = [vp, ctypes.POINTER(Job), vp, ctypes.POINTER(Opts), vp]
     L.migrid_ctx_fft.argtypes = [vp, i32, i32, vp, vp, i64, i32, ctypes.c_double, vp, vp]
+    L.migrid_grid_noalloc.argtypes = [ctypes.POINTER(Job), ctypes.POINTER(Opts), vp]
     _lib = L
     return L
 
@@ -437,6 +438,22 @@ def uvw_to_pixels(uvw, freq, N, cell, nplanes=0, w_max=1.0, stream=None):
                                      nplanes, float(w_max), uv.data_ptr(),
                                      g.data_ptr() if g is not None else None, s))
     return uv, g
+
+
+def grid_noalloc(grid, uv, vals, gcf, N, S, grid_lo=None, stream=None, **kw):
+    """migrid_grid_noalloc on device tensors: Context.grid's work with the
+    device scratch taken from the library's static per-GPU array (no device
+    allocation; jobs beyond its size raise).  Asynchronous on `stream`."""
+    import torch
+    for t in (uv, vals, grid) + ((gcf,) if gcf is not None else ()):
+        if not t.is_contiguous() or not t.is_cuda:
+            raise MigridError(-1, "device tensors must be contiguous and on the GPU")
+    j = make_job(grid, uv, vals, gcf, N, S, grid_lo)
+    o = make_opts(**kw)
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    check(lib().migrid_grid_noalloc(ctypes.byref(j), ctypes.byref(o),
+                                    ctypes.c_void_p(stream.cuda_stream)))
 
 
 def make_job(grid, uv, vals, gcf, N, S, grid_lo=None):

@@ -56,6 +56,8 @@ def test_null_pointers_rejected():
     assert L.migrid_gridGPU_cc(buf.ctypes.data, None, None, 3, 64, buf.ctypes.data, 8, None) == -1
     assert L.migrid_gridGPU_zc(buf.ctypes.data, buf.ctypes.data, buf.ctypes.data, 1, 64, None, 8, None) == -1
     assert L.migrid_ctx_grid(None, None, None, None) == -1
+    assert L.migrid_grid_noalloc(None, None, None) == -1
+    assert "job" in L.migrid_last_error().decode()
 
 
 def test_gcf_size_checked_by_binding():
