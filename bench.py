@@ -493,6 +493,79 @@ def main_image(args, torch, migrid):
     print(json.dumps(out_line), flush=True)
 
 
+def main_ceilings(args, torch, migrid):
+    """Measured ceilings beside the kernels they bound, on the grid workload:
+    LDS read bandwidth (the tiled owner kernel reads one GCF element from LDS
+    per executed tap) and global float atomics at the plain scatter kernel's
+    address distribution (2 S^2 per visibility)."""
+    N, S, npts = args.N, args.S, args.npts
+    uv, vals = gen_clustered(npts, N, S) if args.dist == "clustered" else gen_uniform(npts, N, S)
+    gcf = config_gcf(S)
+    d_uv = torch.from_numpy(uv).cuda()
+    d_vals = torch.from_numpy(vals).cuda()
+    d_gcf = torch.from_numpy(gcf).cuda()
+    lds = {}
+    for eb in (8, 16):
+        for pitch in (0, 96, 100, 104, 108, 112, 116, 120, 124, 97, 99):
+            r = migrid.ceiling_lds(eb, pitch, iters=args.lds_iters)
+            r["tb_per_s"] = r["per_s"] / 1e12
+            pname = "linear" if pitch == 0 else f"owner_block_pitch{pitch}"
+            lds[f"b{8 * eb}_{pname}"] = r
+            print(f"# lds {eb} B {pname}: {r['tb_per_s']:.1f} TB/s ({r['ms']:.3f} ms)",
+                  file=sys.stderr, flush=True)
+    atom = {}
+    ctx = migrid.Context()
+    kern = {}
+    for name, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        g = torch.zeros((N, N, 2), dtype=dt, device="cuda")
+        migrid.ceiling_atomics(g, d_uv, N, S)  # warm-up
+        r = migrid.ceiling_atomics(g, d_uv, N, S)
+        r["atomics_per_vis"] = 2 * S * S
+        atom[name] = r
+        print(f"# atomics {name}: {r}", file=sys.stderr, flush=True)
+        # the plain scatter kernel on the same inputs and grid type
+        uv_in = d_uv if dt == torch.float64 else d_uv.float()
+        vals_in = d_vals if dt == torch.float64 else d_vals.float()
+        gcf_in = d_gcf if dt == torch.float64 else d_gcf.float()
+        _, _, km = time_variant(torch, ctx, lambda: ctx.grid(g, uv_in, vals_in, gcf_in, N, S,
+                                                             strategy="scatter"), 1, 2)
+        kern[f"scatter_{name}"] = float(np.mean(km))
+        del g
+    # the owner kernels on the same workload: executed taps from the bin
+    # geometry (each visibility listed by (ceil((S + 15) / 16))^2 waves of 256
+    # lane-taps when its window is inside the grid), one LDS element per
+    # lane-tap: four 16-byte reads per visibility and wave for FP64 pairs
+    # (ds_read_b128) and two for FP32 pairs (ds_read2_b64, two 8-byte
+    # elements each).  The probe shows LDS time goes per wave instruction
+    # (8- and 16-byte reads take the same time when free of bank conflicts),
+    # so the ceiling is the 16-byte probe's instruction rate at the kernel's
+    # pitch (104 elements, 8 mod 32).
+    waves = ((S + 15 + 15) // 16) ** 2
+    instr_ceiling = lds["b128_owner_block_pitch104"]["per_s"] / (64 * 16)
+    g = torch.empty((N, N, 2), dtype=torch.float64, device="cuda")
+    owner = {}
+    for name, arith, eb, per_vis in (("fp64", migrid.ARITH_NATIVE, 16, 4),
+                                     ("fp32", migrid.ARITH_F32, 8, 2)):
+        _, _, km = time_variant(torch, ctx, lambda: ctx.grid(g, d_uv, d_vals, d_gcf, N, S,
+                                                             strategy="tiled", arith=arith), 2, 5)
+        ms = float(np.mean(km))
+        lds_bytes = float(npts) * waves * 256 * eb
+        instr = float(npts) * waves * per_vis
+        owner[name] = {"kernel_ms": ms, "lds_bytes_est": lds_bytes,
+                       "lds_tb_per_s": lds_bytes / (ms * 1e-3) / 1e12,
+                       "lds_wave_instr_est": instr,
+                       "frac_of_lds_instr_ceiling": instr / (ms * 1e-3) / instr_ceiling,
+                       "executed_taps_per_vis": waves * 256, "useful_taps_per_vis": S * S}
+    out = {"metric": "measured ceilings: LDS read bandwidth and global float atomics",
+           "config": {"workload": f"{N}x{N} grid, support {S}, {npts} {args.dist} visibilities"},
+           "lds": lds, "lds_wave_instr_per_s_ceiling": instr_ceiling, "atomics": atom,
+           "owner_kernel": owner,
+           "scatter_kernel": {k: {"kernel_ms": v,
+                                  "frac_of_atomic_ceiling": atom[k.split("_")[1]]["ms"] / v}
+                              for k, v in kern.items()}}
+    print(json.dumps(out), flush=True)
+
+
 def main_copy(args, torch):
     """Plain device-to-device copy of 1 GiB: the calibration kernel for the
     HBM byte counters (it reads and writes exactly 2^30 bytes per launch)."""
@@ -543,12 +616,14 @@ def main():
     ap.add_argument("--tap-smax", type=int, default=0,
                     help="tiled: largest support gridded one tap per lane (0: 16, -1: never)")
     ap.add_argument("--dist", default="clustered", choices=["clustered", "uniform"])
-    ap.add_argument("--mode", default="grid", choices=["grid", "batch", "serve", "copy", "degrid", "image", "cpu"],
+    ap.add_argument("--mode", default="grid", choices=["grid", "batch", "serve", "copy", "degrid", "image", "cpu",
+                             "ceilings"],
                     help="grid: one 8192^2 job (default); batch: many jobs in one persistent "
                          "launch; serve: Poisson request stream through the serving loop; "
                          "copy: 1 GiB device copy (byte-counter calibration); degrid: the "
                          "adjoint (grid -> visibilities) of the grid workload; image: the "
-                         "grid -> image FFT; cpu: the CPU baselines in full (no GPU)")
+                         "grid -> image FFT; cpu: the CPU baselines in full (no GPU); "
+                         "ceilings: LDS read and global atomic ceilings beside the kernels")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--jobs", type=int, default=64, help="batch/serve: job ids")
     ap.add_argument("--large-jobs", type=int, default=4, help="serve: 8192^2 job ids")
@@ -558,6 +633,7 @@ def main():
                     help="serve: seconds of the trace replayed on the plain FP64 path")
     ap.add_argument("--max-batch", type=int, default=0)
     ap.add_argument("--max-inflight", type=int, default=0)
+    ap.add_argument("--lds-iters", type=int, default=20000, help="ceilings: LDS probe steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported "
@@ -579,6 +655,8 @@ def main():
         return main_degrid(args, torch, migrid)
     if args.mode == "image":
         return main_image(args, torch, migrid)
+    if args.mode == "ceilings":
+        return main_ceilings(args, torch, migrid)
     N, S, npts = args.N, args.S, args.npts
     t0 = time.perf_counter()
     uv, vals = gen_clustered(npts, N, S) if args.dist == "clustered" else gen_uniform(npts, N, S)

@@ -242,6 +242,28 @@ int migrid_batch_destroy(migrid_batch* batch);
 int migrid_autotune(migrid_ctx* ctx, const migrid_job* jobs, int njobs, const migrid_opts* base,
                     migrid_opts* best, double* best_ms, void* stream);
 
+/* ---- measured ceilings ---------------------------------------------------
+ * No counterpart in the reference, which prints only its own kernels'
+ * rates (grid_gpu.cu:789-791).  Synchronous, on the current device.
+ * migrid_ceiling_lds: LDS read bandwidth at the tiled owner kernel's
+ * occupancy (1024-thread workgroups, two per CU), elem_bytes 8 (FP32 pairs)
+ * or 16 (FP64 pairs); pitch 0: linear rows (lane i reads element i), else
+ * the owner kernel's lane block (2 x 2 elements at stride 8) with that row
+ * pitch in elements (96..124 in steps of 4, 97 or 99), shifted per step;
+ * amount = bytes read; per_clk_cu and clock_ghz count s_memtime ticks.
+ * migrid_ceiling_atomics: the plain scatter kernel's global atomics alone
+ * (2 per tap, the job's visibilities and grid; constants added, no table
+ * reads): amount = atomics issued, assuming every tap on the grid. */
+typedef struct migrid_ceiling {
+    double ms;         /* kernel time of the measured launch (HIP events) */
+    double amount;     /* bytes read (LDS) or atomics issued */
+    double per_s;      /* amount per second */
+    double per_clk_cu; /* LDS: bytes per s_memtime tick per CU */
+    double clock_ghz;  /* LDS: s_memtime ticks per ns over the loop */
+} migrid_ceiling;
+int migrid_ceiling_lds(int32_t elem_bytes, int32_t pitch, int32_t iters, migrid_ceiling* res);
+int migrid_ceiling_atomics(const migrid_job* job, migrid_ceiling* res);
+
 /* Number of bins per axis for migrid_ctx_bin_sort. */
 int64_t migrid_bins_per_axis(int64_t img_dim, int64_t gcf_dim, int32_t bin);
 

@@ -91,6 +91,26 @@ class ServerStats(ctypes.Structure):
         "jobs_resident", "queued", "running")] + [("gpu_ms", ctypes.c_double)]
 
 
+class Ceiling(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("ms", "amount", "per_s", "per_clk_cu", "clock_ghz")]
+
+
+def ceiling_lds(elem_bytes, pitch, iters=20000):
+    """LDS read bandwidth probe (migrid_ceiling_lds) as a dict; pitch 0 reads
+    linear rows, else the owner kernel's lane block at that row pitch."""
+    c = Ceiling()
+    check(lib().migrid_ceiling_lds(elem_bytes, pitch, iters, ctypes.byref(c)))
+    return {n: getattr(c, n) for n, _ in Ceiling._fields_}
+
+
+def ceiling_atomics(grid, uv, N, S):
+    """The scatter kernel's atomics alone (migrid_ceiling_atomics) as a dict."""
+    j = make_job(grid, uv, uv, None, N, S)
+    c = Ceiling()
+    check(lib().migrid_ceiling_atomics(ctypes.byref(j), ctypes.byref(c)))
+    return {n: getattr(c, n) for n, _ in Ceiling._fields_}
+
+
 # Every symbol include/migrid.h declares (checked by the CPU test suite).
 EXPORTS = [
     "migrid_opts_default", "migrid_last_error", "migrid_version",
@@ -107,6 +127,7 @@ EXPORTS = [
     "migrid_server_get_stats", "migrid_server_destroy",
     "migrid_uvw_to_pixels", "migrid_ctx_grid_indexed", "migrid_read_uvw_text",
     "migrid_ctx_degrid", "migrid_ctx_fft", "migrid_grid_noalloc",
+    "migrid_ceiling_lds", "migrid_ceiling_atomics",
 ]
 
 _lib = None
@@ -171,6 +192,8 @@ def lib():
     L.migrid_ctx_degrid.argtypes = [vp, ctypes.POINTER(Job), vp, ctypes.POINTER(Opts), vp]
     L.migrid_ctx_fft.argtypes = [vp, i32, i32, vp, vp, i64, i32, ctypes.c_double, vp, vp]
     L.migrid_grid_noalloc.argtypes = [ctypes.POINTER(Job), ctypes.POINTER(Opts), vp]
+    L.migrid_ceiling_lds.argtypes = [i32, i32, i32, ctypes.POINTER(Ceiling)]
+    L.migrid_ceiling_atomics.argtypes = [ctypes.POINTER(Job), ctypes.POINTER(Ceiling)]
     _lib = L
     return L
 

@@ -60,6 +60,12 @@ def test_owner_indexing_stays_in_bounds(N, S, T):
                     if not wave_on:
                         continue
                     assert byh - byl + 1 <= 5
+                    # the visibilities whose window reaches the block's pixels on
+                    # the grid: each must be listed for this wave exactly once
+                    xe, ye = min(Xw + 15, N - 1), min(Yw + 15, N - 1)
+                    reach = ((q[:, 0] >= Xw) & (q[:, 0] - S + 1 <= xe) &
+                             (q[:, 1] >= Yw) & (q[:, 1] - S + 1 <= ye))
+                    listed = np.zeros(count, np.int64)
                     for k in range(64):
                         for by in range(byl, byh + 1):
                             key = (k * nb + by) * nb
@@ -69,6 +75,7 @@ def test_owner_indexing_stays_in_bounds(N, S, T):
                             assert 0 <= r0 <= r1 <= count
                             if r0 == r1:
                                 continue
+                            listed[r0:r1] += 1
                             dx = q[r0:r1, 0] - Xw
                             dy = q[r0:r1, 1] - Yw
                             assert dx.min() >= 0 and dy.min() >= 0
@@ -80,6 +87,7 @@ def test_owner_indexing_stays_in_bounds(N, S, T):
                             assert col_lo >= 0 and col_hi < R and col_hi < P
                             assert row_lo >= 0 and row_hi < R
                             assert row_hi * P + col_hi < R * P
+                    assert (listed[reach] == 1).all() and listed.max() <= 1
 
 
 def tile_counts(offs, N, S, T):
