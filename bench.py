@@ -227,12 +227,65 @@ def batch_jobs(rng, njobs):
     return [(int(a), int(b), int(c)) for a, b, c in zip(Ns, Ss, n)]
 
 
+def job_roofline_ms(N, S, n, accumulate, fp32=True):
+    """The roofline time of one gridding job: the larger of its VALU time
+    (8 S^2 flops per visibility at the vector peak) and its HBM time
+    (32 B of FP64 inputs per visibility, the 64 S^2 complex128 table, the
+    N^2 complex128 grid written once, or read and written when accumulating)."""
+    flops = 8.0 * S * S * n
+    bytes_ = 32.0 * n + 64.0 * S * S * 16 + 16.0 * N * N * (2 if accumulate else 1)
+    peak = PEAK_FP32_TF if fp32 else PEAK_FP64_TF
+    return max(flops / (peak * 1e12), bytes_ / (PEAK_HBM_GBS * 1e9)) * 1e3, flops
+
+
+def mix_roofline(parts, ms, accumulate):
+    """roofline of a mix of jobs (N, S, n) run in ms: achieved TFLOP/s against
+    the FP32 vector peak, and the fraction of the per-job roofline time
+    (time-weighted: the jobs' roofline times summed, over the measured time)."""
+    t_roof = flops = 0.0
+    for N, S, n in parts:
+        t, f = job_roofline_ms(N, S, n, accumulate)
+        t_roof += t
+        flops += f
+    achieved = flops / (ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TF, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TF, "traffic": None,
+            "per_job_roofline_ms": t_roof, "frac_of_per_job_roofline": t_roof / ms,
+            "flops": flops,
+            "rule": "per job max(8 S^2 n / 157.3 TFLOP/s, bytes / 8 TB/s), summed over jobs"}
+
+
+def cpu_mix_baseline(parts_host, gcfs, sample, what):
+    """gridCPU restatement, single thread, on the first `sample` visibilities
+    of each job; the mix's rate is its visibilities over the jobs' sampled
+    times scaled to their sizes."""
+    O = oracle_mod()
+    t_tot = 0.0
+    n_tot = 0
+    sampled = 0
+    for (uv, vals, N, S) in parts_host:
+        m = min(sample, uv.shape[0])
+        if m == 0:
+            continue
+        t0 = time.perf_counter()
+        O.grid_zz(uv[:m], vals[:m], N, gcfs[S], S)
+        dt = time.perf_counter() - t0
+        t_tot += dt * uv.shape[0] / m
+        n_tot += uv.shape[0]
+        sampled += m
+    return dict({"value": n_tot / t_tot / 1e6, "unit": "Mvis/s", "cores": 1, "kind": "port",
+                 "sample": f"first {sample} visibilities of each of the {len(parts_host)} {what}, "
+                           f"{sampled} in all, through the single-threaded gridCPU restatement; each "
+                           f"job's time scaled to its size and summed"}, **host_cpu())
+
+
 def main_batch(args, torch, migrid):
     rng = np.random.default_rng(args.seed)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(args.seed)
     shapes = batch_jobs(rng, args.jobs)
-    gcfs = {S: torch.from_numpy(config_gcf(S)).cuda() for S in (8, 16, 32, 64)}
+    gcf_h = {S: config_gcf(S) for S in (8, 16, 32, 64)}
+    gcfs = {S: torch.from_numpy(g).cuda() for S, g in gcf_h.items()}
     jobs = []
     for N, S, n in shapes:
         uv, vals = gen_clustered_dev(torch, n, N, S, 800.0 * N / 8192, gen)
@@ -262,17 +315,43 @@ def main_batch(args, torch, migrid):
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     per = [e0.elapsed_time(e1) for e0, e1 in evs]
-    tiled = [j["grid"].clone() for j in jobs[:8]]
+    # the checked result: one more replay into NaN-filled grids (untimed)
+    for j in jobs:
+        j["grid"].fill_(float("nan"))
+    torch.cuda.synchronize()
+    b.graph_launch(stream)
+    torch.cuda.synchronize()
+    host = [(j["uv"].cpu().numpy(), j["vals"].cpu().numpy(), j["N"], j["S"]) for j in jobs]
+    err = {"rel_max_vs_oracle": 0.0, "rel_rms_vs_oracle": 0.0}
+    base_thr = None
+    if not args.no_cpu:
+        # every job through the OpenMP gridCPU restatement: the error oracle,
+        # and the threaded CPU figure for the whole batch
+        O = oracle_mod()
+        t_cpu, used = 0.0, 0
+        for j, (uv, vals, N, S) in zip(jobs, host):
+            t1 = time.perf_counter()
+            ref, used = O.grid_zz_omp(uv, vals, N, gcf_h[S], S, omp_threads())
+            t_cpu += time.perf_counter() - t1
+            r = torch.from_numpy(np.ascontiguousarray(np.moveaxis(ref, 0, -1))).cuda()
+            d = torch.linalg.vector_norm(j["grid"] - r, dim=-1)
+            m = torch.linalg.vector_norm(r, dim=-1)
+            err["rel_max_vs_oracle"] = max(err["rel_max_vs_oracle"], float(d.max() / m.max()))
+            err["rel_rms_vs_oracle"] = max(err["rel_rms_vs_oracle"],
+                                           float(torch.sqrt((d * d).sum() / (m * m).sum())))
+            del r, d, m, ref
+        base_thr = dict({"value": total / t_cpu / 1e6, "unit": "Mvis/s", "cores": int(used),
+                         "kind": "port",
+                         "sample": f"every job of the batch ({total} visibilities), row-band "
+                                   f"OpenMP form of the gridCPU restatement on {used} threads, "
+                                   f"{t_cpu:.1f} s"}, **host_cpu())
+        print(f"# batch oracle: {base_thr['sample']}; worst job {err}", file=sys.stderr, flush=True)
     # the plain FP64 path: one scatter launch per job
     t0 = time.perf_counter()
     for j in jobs:
         ctx.grid(j["grid"], j["uv"], j["vals"], j["gcf"], j["N"], j["S"], strategy="scatter")
     torch.cuda.synchronize()
     plain_ms = (time.perf_counter() - t0) * 1e3
-    err = 0.0
-    for g, j in zip(tiled, jobs[:8]):
-        d = torch.linalg.vector_norm(g - j["grid"], dim=-1).max()
-        err = max(err, float(d / torch.linalg.vector_norm(j["grid"], dim=-1).max()))
     out = {
         "metric": "Mvis/s, 64 uneven gridding jobs in one hipGraph-captured persistent launch",
         "value": total / (ms * 1e-3) / 1e6, "unit": "Mvis/s", "n_gpus": 1, "steps": args.steps,
@@ -287,8 +366,13 @@ def main_batch(args, torch, migrid):
                      "seconds": tune_s},
         "fp64_plain_one_launch_per_job": {"ms": plain_ms,
                                           "mvis_per_s": total / (plain_ms * 1e-3) / 1e6},
-        "rel_max_vs_fp64_plain": err,
+        "err_worst_job": err if not args.no_cpu else None,
+        "roofline": mix_roofline(shapes, ms, accumulate=False),
     }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_mix_baseline(host, gcf_h, args.cpu_sample // 25,
+                                               "jobs of the batch")
+        out["cpu_baseline_threads"] = base_thr
     print(json.dumps(out), flush=True)
 
 
@@ -297,7 +381,8 @@ def main_serve(args, torch, migrid):
     rng = np.random.default_rng(args.seed)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(args.seed)
-    gcfs = {S: torch.from_numpy(config_gcf(S)).cuda() for S in (8, 16, 32, 64)}
+    gcf_h = {S: config_gcf(S) for S in (8, 16, 32, 64)}
+    gcfs = {S: torch.from_numpy(g).cuda() for S, g in gcf_h.items()}
     small = batch_jobs(rng, args.jobs)  # (N, S, max vis) per small job id
     pools = {}
     for jid, (N, S, _) in enumerate(small):
@@ -334,12 +419,19 @@ def main_serve(args, torch, migrid):
     srv.drop_job(-1)
     st0 = srv.stats()
     tickets = []
+    kept = set()
     t_start = time.perf_counter()
     for t, jid, n, off in trace:
         now = time.perf_counter() - t_start
         if t > now:
             time.sleep(t - now)
-        tickets.append(srv.submit(jid, *req_args(jid, n, off)))
+        # the large job ids repeat their coordinates: the first request keeps
+        # its sort, later ones reuse it
+        flags = {}
+        if jid >= 1000:
+            flags = {"same_uv": True} if jid in kept else {"keep_sort": True}
+            kept.add(jid)
+        tickets.append(srv.submit(jid, *req_args(jid, n, off), **flags))
         if len(tickets) % 2000 == 0:
             print(f"# serve: {len(tickets)} submitted", file=sys.stderr, flush=True)
     lat = [srv.wait(tk) for tk in tickets]
@@ -347,7 +439,7 @@ def main_serve(args, torch, migrid):
     st = srv.stats()
     srv.close()
     batches = st["batches"] - st0["batches"]
-    gpu_ms = st["gpu_ms"] - st0["gpu_ms"]
+    busy_ms = st["gpu_busy_ms"] - st0["gpu_busy_ms"]
     # the plain FP64 path on a prefix of the same trace: FIFO, one scatter
     # launch per request, accumulating into per-job grids
     prefix = [r for r in trace if r[0] < args.plain_duration]
@@ -367,6 +459,7 @@ def main_serve(args, torch, migrid):
         plain_lat.append((time.perf_counter() - t_start - t) * 1e3)
     plain_wall = time.perf_counter() - t_start
     plain_vis = sum(n for _, _, n, _ in prefix)
+    parts = [(pools[jid][2], pools[jid][3], n) for _, jid, n, _ in trace]
     out = {
         "metric": "Mvis/s and p99 latency of the serving loop, Poisson 200 req/s",
         "value": total / wall / 1e6, "unit": "Mvis/s", "n_gpus": 1, "steps": len(trace),
@@ -380,13 +473,39 @@ def main_serve(args, torch, migrid):
                    "requests": len(trace), "npts_total": total},
         "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "max_ms": max(lat),
         "batches": batches, "mean_requests_per_batch": len(trace) / max(batches, 1),
-        "gpu_busy_frac": gpu_ms / (wall * 1e3),
+        "gpu_busy_frac": busy_ms / (wall * 1e3),
+        "gpu_overlap_ms": st["overlap_ms"] - st0["overlap_ms"],
+        "batch_device_ms_summed": st["gpu_ms"] - st0["gpu_ms"],
+        "sorts_reused": st["sorts_reused"] - st0["sorts_reused"],
         "offered_mvis_per_s": total / args.duration / 1e6,
+        # against the device-busy time: how close the served work runs to its
+        # per-request roofline while the GPU is working
+        "roofline": mix_roofline(parts, busy_ms, accumulate=True),
         "fp64_plain_one_launch_per_request": {
             "requests": len(prefix), "trace_seconds": args.plain_duration,
             "mvis_per_s": plain_vis / plain_wall / 1e6,
             "p50_ms": pct(plain_lat, 50), "p99_ms": pct(plain_lat, 99)},
     }
+    if not args.no_cpu:
+        # gridCPU restatement, single thread, on the first visibilities of each
+        # job id, each job id's time scaled to the visibilities the trace sends it
+        O = oracle_mod()
+        load = {}
+        for _, jid, n, _ in trace:
+            load[jid] = load.get(jid, 0) + n
+        m = args.cpu_sample // 25
+        t_tot = 0.0
+        for jid in sorted(load):
+            uv, vals, N, S = pools[jid]
+            uv_h, vals_h = uv[:m].cpu().numpy(), vals[:m].cpu().numpy()
+            t1 = time.perf_counter()
+            O.grid_zz(uv_h, vals_h, N, gcf_h[S], S)
+            t_tot += (time.perf_counter() - t1) * load[jid] / m
+        cb = dict({"value": total / t_tot / 1e6, "unit": "Mvis/s", "cores": 1, "kind": "port",
+                   "sample": f"first {m} visibilities of each of the {len(load)} job ids through "
+                             "the single-threaded gridCPU restatement, each job id's time scaled "
+                             "to the visibilities the trace sends it"}, **host_cpu())
+        out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
 
@@ -405,11 +524,13 @@ def main_degrid(args, torch, migrid):
     grid = torch.randn((N, N, 2), dtype=torch.float64, device="cuda", generator=gen)
     out = torch.empty((npts, 2), dtype=torch.float64, device="cuda")
     ctx = migrid.Context()
-    nchk = min(npts, 4000)
+    nchk = min(npts, 12000)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # CPU restatement, the checker of the first visibilities
     hgrid = grid.permute(2, 0, 1).contiguous().cpu().numpy()
+    t_cpu = time.perf_counter()
     ref = O.degrid_zz(uv[:nchk], hgrid, N, gcf, S)
+    t_cpu = time.perf_counter() - t_cpu
     results = {}
     for name, arith in (("fp64", migrid.ARITH_NATIVE), ("fp32", migrid.ARITH_F32)):
         def step(arith=arith):
@@ -446,6 +567,11 @@ def main_degrid(args, torch, migrid):
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TF,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TF, "traffic": None,
                      "kernel": "degridding kernel", "flops_per_launch": flops},
+        "cpu_baseline": dict({"value": nchk / t_cpu / 1e6, "unit": "Mvis/s", "cores": 1,
+                              "kind": "port",
+                              "sample": f"first {nchk} visibilities of the same workload through "
+                                        f"the oracle's single-threaded degridder (the adjoint of "
+                                        f"the gridCPU restatement), {t_cpu:.1f} s"}, **host_cpu()),
     }
     print(json.dumps(out_line), flush=True)
 
@@ -490,6 +616,20 @@ def main_image(args, torch, migrid):
                      "kernel": "pre pass + hipFFT + post pass",
                      "bytes_per_launch": 2 * N * N * 16},
     }
+    if not args.no_cpu:
+        # the oracle's numpy restatement of the centred transform, same grid
+        O = oracle_mod()
+        g = torch.randn((N, N, 2), dtype=torch.float64, device="cuda", generator=gen)
+        gh = g[..., 0].cpu().numpy() + 1j * g[..., 1].cpu().numpy()
+        del g
+        t_cpu = time.perf_counter()
+        O.grid_to_image(gh, 1.0 / (N * N), np.linspace(0.5, 1.0, N))
+        t_cpu = time.perf_counter() - t_cpu
+        out_line["cpu_baseline"] = dict({"value": 1.0 / t_cpu, "unit": "images/s", "cores": 1,
+                                         "kind": "port",
+                                         "sample": f"one {N}x{N} complex128 transform with taper "
+                                                   f"through numpy's FFT (the oracle's "
+                                                   f"restatement), {t_cpu:.1f} s"}, **host_cpu())
     print(json.dumps(out_line), flush=True)
 
 

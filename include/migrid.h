@@ -335,14 +335,19 @@ int migrid_read_uvw_text(const char* path, int64_t img_dim, int64_t max_rows, do
 /* ---- serving loop (continuous batching) -----------------------------------
  * No counterpart in the reference, whose only entry is the blocking
  * per-job gridGPU call (grid_gpu.cu:675-819).  A server owns a worker thread
- * and two streams on one device.  Requests carry a job id; each job id owns a
- * resident N x N grid (interleaved complex, out_type) that starts at zero and
- * accumulates every request submitted for it until migrid_server_drop_job.
- * Requests are admitted in arrival order while their device footprint fits
- * hbm_budget; up to max_inflight batches of up to max_batch requests (one per
- * job id) are queued on the device, each batch one persistent launch of the
- * tiled gridder with cfg.opts (strategy and accumulate are forced).
- * Thread-safe: any thread may submit, poll, wait, read and drop. */
+ * on one device, a copy stream and one stream per batch slot.  Requests carry
+ * a job id; each job id owns a resident N x N grid (interleaved complex,
+ * out_type) that starts at zero and accumulates every request submitted for
+ * it until migrid_server_drop_job.  Requests are admitted in arrival order
+ * while their device footprint fits hbm_budget, and everything admitted forms
+ * the next batch (up to max_batch requests, one per job id) as soon as a slot
+ * is free: up to max_inflight batches run on the device at once, each one
+ * persistent launch of the tiled gridder with cfg.opts (strategy and
+ * accumulate are forced) on its own stream.  Batches that write the same grid
+ * run in submission order; others overlap.  A job can keep its bin sort
+ * resident (MIGRID_SUBMIT_KEEP_SORT) so later requests with the same
+ * coordinates (MIGRID_SUBMIT_SAME_UV) skip the sort and only pack their new
+ * values.  Thread-safe: any thread may submit, poll, wait, read and drop. */
 typedef struct migrid_server migrid_server;
 
 typedef struct migrid_server_cfg {
@@ -350,7 +355,7 @@ typedef struct migrid_server_cfg {
     int32_t in_type;      /* MIGRID_F64 or MIGRID_F32 uv/vals/gcf */
     int32_t out_type;     /* resident grid element type */
     int32_t max_batch;    /* requests per launch (0: 64) */
-    int32_t max_inflight; /* batches queued on the device, 1..8 (0: 2) */
+    int32_t max_inflight; /* batches on the device at once, 1..8 (0: 3) */
     int32_t reserved0;
     int64_t hbm_budget;   /* bytes the server may hold (0: 90% of free HBM) */
     migrid_opts opts;     /* arithmetic, tile, tuning; GCF mode */
@@ -363,10 +368,22 @@ typedef struct migrid_server_stats {
     int64_t queued;  /* submitted or admitted, not yet on the device */
     int64_t running; /* in batches on the device */
     double gpu_ms;   /* summed device time of completed batches */
+    double gpu_busy_ms;  /* time with at least one batch on the device (union) */
+    double overlap_ms;   /* time with two or more batches on the device */
+    int64_t sorts_reused; /* requests that took their job's kept sort */
 } migrid_server_stats;
 
 #define MIGRID_SUBMIT_HOST 0   /* uv/vals/gcf in host memory, copied at submit */
 #define MIGRID_SUBMIT_DEVICE 1 /* device pointers, valid until the request completes */
+/* Keep this request's bin sort (its permutation and CSR offsets, about
+ * 4 * npts + 4 * 64 * ((N + S - 2) / 16 + 1)^2 bytes) resident with its job,
+ * replacing a sort kept before. */
+#define MIGRID_SUBMIT_KEEP_SORT 2
+/* This request's coordinates are those of the sort its job keeps (the same
+ * count, support and values: the caller's promise): the sort is reused and
+ * only the values are packed.  The request fails (EINVAL) when its job keeps
+ * no sort of that count and support. */
+#define MIGRID_SUBMIT_SAME_UV 4
 
 void migrid_server_cfg_default(migrid_server_cfg* cfg);
 int migrid_server_create(const migrid_server_cfg* cfg, migrid_server** server);

@@ -71,7 +71,8 @@ int job_args(const migrid_job* j, const migrid_opts& o, GridArgs& a)
 
 // Clears (unless accumulating), prepares and runs one planned launch.
 int run_plan(const TiledPlan& p, const std::vector<GridArgs>& args, const migrid_opts& o,
-             char* arena, void* bins, Profiler* prof, hipStream_t s)
+             char* arena, void* bins, Profiler* prof, hipStream_t s, int n_sorted = -1,
+             const void* const* kept = nullptr)
 {
     // fresh grids: the plan's own passes clear what the owner kernel does not
     // store, except grids shared by several jobs, cleared here once each
@@ -86,7 +87,7 @@ int run_plan(const TiledPlan& p, const std::vector<GridArgs>& args, const migrid
             }
         }
     }
-    MG_TRY(tiled_prepare(p, arena, bins, s));
+    MG_TRY(tiled_prepare(p, arena, bins, s, n_sorted, kept));
     if (prof) MG_TRY(prof->begin(s));
     MG_TRY(tiled_execute(p, arena, s));
     if (prof) MG_TRY(prof->end(s));

@@ -1382,9 +1382,48 @@ int tiled_write_descs(const TiledPlan& p, char* arena, const std::vector<JobDesc
     return put_table((JobDesc*)(arena + p.off_desc), d.data(), (int)d.size(), s);
 }
 
-int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_t s)
+// dst[i] = src[i] + delta: a kept sort moved between a job's own numbering
+// and its place in a launch (delta = +/- the job's first element).
+static __global__ __launch_bounds__(256) void k_rebase(int32_t* __restrict__ dst,
+                                                const int32_t* __restrict__ src, int64_t n,
+                                                int32_t delta)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i] + delta;
+}
+
+static int launch_rebase(int32_t* dst, const int32_t* src, int64_t n, int64_t delta, hipStream_t s)
+{
+    if (n <= 0) return MIGRID_OK;
+    const int64_t b = (n + 255) / 256;
+    hipLaunchKernelGGL(k_rebase, dim3((unsigned)(b < 8192 ? b : 8192)), dim3(256), 0, s, dst, src,
+                       n, (int32_t)delta);
+    MG_HIP(hipGetLastError());
+    return MIGRID_OK;
+}
+
+size_t tiled_sort_bytes(const TiledJob& t)
+{
+    return align256((size_t)(t.a.npts > 0 ? t.a.npts : 1) * 4) + (size_t)(t.bg.nkeys + 1) * 4;
+}
+
+int tiled_keep_sort(const TiledPlan& p, const char* arena, int j, void* dst, hipStream_t s)
+{
+    const TiledJob& t = p.jobs[j];
+    const int32_t* perm = (const int32_t*)(arena + p.off_perm) + t.e_base;
+    const int32_t* offs = (const int32_t*)(arena + p.off_offs) + t.k_base;
+    int32_t* d_perm = (int32_t*)dst;
+    int32_t* d_offs = (int32_t*)((char*)dst + align256((size_t)(t.a.npts > 0 ? t.a.npts : 1) * 4));
+    MG_TRY(launch_rebase(d_perm, perm, t.a.npts, -t.e_base, s));
+    return launch_rebase(d_offs, offs, t.bg.nkeys + 1, -t.e_base, s);
+}
+
+int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_t s, int n_sorted,
+                  const void* const* kept)
 {
     const int n = (int)p.jobs.size();
+    if (n_sorted < 0 || n_sorted > n) n_sorted = n;
     QueueState* qs = (QueueState*)(arena + p.off_qs);
     Item* items = (Item*)(arena + p.off_items);
     int32_t* perm = (int32_t*)(arena + p.off_perm);
@@ -1430,7 +1469,27 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
         MG_TRY(launch_convert(dcj, (int)cj.size(), p.max_gcf, p.key.stage,
                               p.key.in_type == MIGRID_F64, s));
     }
-    MG_TRY(bin_sort(dbj, n, p.max_npts, p.E, p.K, p.key.in_type, bin_scratch, perm, offs, s));
+    // the jobs sorted here come first; the others' kept sorts are moved to
+    // their places in this launch
+    if (n_sorted == n) {
+        MG_TRY(bin_sort(dbj, n, p.max_npts, p.E, p.K, p.key.in_type, bin_scratch, perm, offs, s));
+    } else {
+        if (n_sorted > 0) {
+            const TiledJob& first = p.jobs[n_sorted];
+            int64_t mx = 0;
+            for (int j = 0; j < n_sorted; j++) mx = std::max<int64_t>(mx, p.jobs[j].a.npts);
+            MG_TRY(bin_sort(dbj, n_sorted, mx, first.e_base, first.k_base, p.key.in_type,
+                            bin_scratch, perm, offs, s));
+        }
+        for (int j = n_sorted; j < n; j++) {
+            const TiledJob& t = p.jobs[j];
+            const int32_t* k_perm = (const int32_t*)kept[j - n_sorted];
+            const int32_t* k_offs = (const int32_t*)((const char*)k_perm +
+                                                     align256((size_t)(t.a.npts > 0 ? t.a.npts : 1) * 4));
+            MG_TRY(launch_rebase(perm + t.e_base, k_perm, t.a.npts, t.e_base, s));
+            MG_TRY(launch_rebase(offs + t.k_base, k_offs, t.bg.nkeys + 1, t.e_base, s));
+        }
+    }
     if (p.E > 0) {
         void* rec = arena + p.off_rec;
         if (p.key.in_type == MIGRID_F64) {

@@ -1,20 +1,24 @@
 // Serving loop: continuous batching of gridding requests on one GPU.
 //
 // The reference has no serving layer (one blocking gridGPU call per job,
-// grid_gpu.cu:675-819).  Here a worker thread owns two streams:
-//   * copy stream: zeroing of new resident grids and uploads of requests
+// grid_gpu.cu:675-819).  Here a worker thread owns
+//   * a copy stream: zeroing of new resident grids and uploads of requests
 //     submitted from host memory (already copied into pinned staging);
-//   * compute stream: batches -- per-job bin sort, record packing and work
-//     items, then ONE persistent owner-kernel launch for the whole batch.
+//   * max_inflight batch slots, each with its own stream, arena and bin
+//     scratch: a batch is the per-job bin sort, record packing and work
+//     items, then ONE persistent owner-kernel launch for all its jobs.
 // Requests are admitted in arrival order while their device footprint (a new
 // job's resident grid, uploaded inputs, work arena estimate) fits the HBM
-// budget.  Up to max_inflight batches are queued on the compute stream, each
-// with its own arena, so the next batch is enqueued behind the running one
-// and the device never waits for the host between batches; everything
-// admitted when a slot frees up (at most one request per job id, so a batch
-// never writes one grid twice) forms the next batch.  Stream order serialises
-// batches, so two requests for the same grid in consecutive batches
-// accumulate in order.  Grids stay resident per job id until dropped.
+// budget; everything admitted (at most one request per job id, so a batch
+// never writes one grid twice) forms the next batch as soon as a slot is
+// free, and goes to the device at once on that slot's stream.  Batches of
+// different slots run concurrently -- the next batch's sort and packing fill
+// the tail of the running persistent kernel, and its kernel takes the CUs the
+// other one leaves -- while a batch that writes a grid waits (stream event)
+// for the previous batch that wrote it, so requests for one grid accumulate
+// in submission order.  Grids stay resident per job id until dropped, and so
+// does a job's bin sort when its request asks to keep it: requests that
+// repeat its coordinates then skip the sort and only pack their values.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,7 +42,8 @@ using namespace migrid;
 namespace migrid {
 int job_args(const migrid_job* j, const migrid_opts& o, GridArgs& a);
 int run_plan(const TiledPlan& p, const std::vector<GridArgs>& args, const migrid_opts& o,
-             char* arena, void* bins, Profiler* prof, hipStream_t s);
+             char* arena, void* bins, Profiler* prof, hipStream_t s, int n_sorted,
+             const void* const* kept);
 }  // namespace migrid
 
 namespace {
@@ -53,6 +58,7 @@ struct Request {
     const void* vals = nullptr;
     const void* gcf = nullptr;
     bool host = false;
+    int32_t flags = 0;       // MIGRID_SUBMIT_*
     bool counted = false;    // admitted: counted in its job's inflight
     void* pinned = nullptr;  // host submissions: uv | vals | gcf
     void* d_in = nullptr;    // their device copy
@@ -68,6 +74,11 @@ struct Job {
     int64_t N = 0;
     int64_t inflight = 0;  // requests admitted, not completed
     int readers = 0;       // host copies of the grid in progress
+    int last_slot = -1;    // slot of the last batch that wrote the grid
+    uint64_t last_seq = 0; // and its sequence number
+    void* sort = nullptr;  // kept bin sort (tiled_keep_sort layout)
+    size_t sort_bytes = 0;
+    int64_t sort_npts = -1, sort_S = 0;  // what the kept sort is of
 };
 
 struct Done {
@@ -78,6 +89,8 @@ struct Done {
 // One batch slot: its own arena and bin scratch, reused batch after batch.
 struct Slot {
     std::vector<Request> reqs;
+    hipStream_t stream = nullptr;
+    uint64_t seq = 0;  // sequence number of the batch last launched here
     hipEvent_t t0 = nullptr, t1 = nullptr;
     void* arena = nullptr;
     size_t arena_bytes = 0;
@@ -97,15 +110,18 @@ struct migrid_server {
     std::deque<Request> pending;    // submitted, not admitted
     std::vector<Request> admitted;  // admitted, not yet in a batch
     std::vector<Slot> slots;
-    std::deque<int> busy;  // slots on the GPU, in launch order
+    std::vector<int> busy;  // slots on the GPU
     std::vector<int> idle;
     std::map<int64_t, Job> jobs;
     std::map<int64_t, Done> done;
     int64_t released_below = 1;  // statuses of tickets below this were released
     int64_t next_ticket = 1;
-    size_t hbm_used = 0;   // resident grids + uploaded inputs
+    size_t hbm_used = 0;   // resident grids, kept sorts and uploaded inputs
     size_t work_used = 0;  // admission estimates of admitted and running work
-    hipStream_t s_copy = nullptr, s_comp = nullptr;
+    hipStream_t s_copy = nullptr;
+    uint64_t batch_seq = 0;
+    hipEvent_t t_ref = nullptr;  // the origin of the batches' device intervals
+    std::vector<std::pair<double, double>> intervals;  // completed batches, ms from t_ref
     migrid_server_stats st;
     char last_err[256] = "";
 };
@@ -135,8 +151,8 @@ size_t work_bytes(int64_t npts, int64_t N, int64_t S)
 
 void release_inputs(migrid_server* s, Request& r)
 {
-    if (r.d_in) {
-        (void)hipFreeAsync(r.d_in, s->s_comp);
+    if (r.d_in) {  // its batch has completed
+        (void)hipFreeAsync(r.d_in, s->s_copy);
         s->hbm_used -= r.in_bytes;
         r.d_in = nullptr;
     }
@@ -317,7 +333,30 @@ int launch_batch(migrid_server* s, int si, std::vector<Request>& take)
         }
     }
     s->admitted = std::move(keep);
+    // a request that reuses its job's kept sort must match it (the job's
+    // earlier requests are already launched); one that does not fails alone
+    {
+        std::vector<Request> ok;
+        for (Request& r : take) {
+            const Job& j = s->jobs[r.job_id];
+            if ((r.flags & MIGRID_SUBMIT_SAME_UV) && !(j.sort_npts == r.npts && j.sort_S == r.S)) {
+                snprintf(s->last_err, sizeof s->last_err,
+                         "job %lld keeps no sort of %lld visibilities at S=%lld", (long long)r.job_id,
+                         (long long)r.npts, (long long)r.S);
+                finish(s, r, MIGRID_EINVAL);
+                continue;
+            }
+            ok.push_back(r);
+        }
+        take = std::move(ok);
+    }
     if (take.empty()) return MIGRID_OK;
+    // the jobs sorted in this batch first, those taking a kept sort last
+    std::stable_partition(take.begin(), take.end(),
+                          [](const Request& r) { return !(r.flags & MIGRID_SUBMIT_SAME_UV); });
+    int n_sorted = 0;
+    while (n_sorted < (int)take.size() && !(take[n_sorted].flags & MIGRID_SUBMIT_SAME_UV))
+        n_sorted++;
     migrid_opts o = s->cfg.opts;
     o.strategy = MIGRID_TILED;
     o.accumulate = 1;  // resident grids accumulate
@@ -343,40 +382,91 @@ int launch_batch(migrid_server* s, int si, std::vector<Request>& take)
     // the slot is idle: nothing on the device uses its buffers
     MG_TRY(ensure(sl.arena, sl.arena_bytes, p.arena_bytes));
     MG_TRY(ensure(sl.bins, sl.bins_bytes, std::max<size_t>(p.bin_bytes, 256)));
-    for (const Request& r : take) MG_HIP(hipStreamWaitEvent(s->s_comp, r.ready, 0));
-    MG_HIP(hipEventRecord(sl.t0, s->s_comp));
+    // room for the sorts this batch keeps (a buffer too small is replaced;
+    // hipFree waits for the device, so no batch still reads the old one)
+    for (size_t i = 0; i < take.size(); i++) {
+        if (!(take[i].flags & MIGRID_SUBMIT_KEEP_SORT) || (take[i].flags & MIGRID_SUBMIT_SAME_UV))
+            continue;
+        Job& j = s->jobs[take[i].job_id];
+        const size_t b = tiled_sort_bytes(p.jobs[i]);
+        if (j.sort_bytes < b) {
+            if (j.sort) {
+                MG_HIP(hipFree(j.sort));
+                s->hbm_used -= j.sort_bytes;
+                j.sort = nullptr;
+                j.sort_bytes = 0;
+            }
+            MG_HIP(hipMalloc(&j.sort, b));
+            j.sort_bytes = b;
+            s->hbm_used += b;
+        }
+    }
+    // wait for the uploads, and for the previous batch that wrote each grid
+    // (it also wrote any kept sort this batch reads)
+    for (const Request& r : take) {
+        MG_HIP(hipStreamWaitEvent(sl.stream, r.ready, 0));
+        const Job& j = s->jobs[r.job_id];
+        if (j.last_slot >= 0 && j.last_slot != si && s->slots[j.last_slot].seq == j.last_seq)
+            MG_HIP(hipStreamWaitEvent(sl.stream, s->slots[j.last_slot].t1, 0));
+    }
+    std::vector<const void*> kept;
+    for (size_t i = n_sorted; i < take.size(); i++) kept.push_back(s->jobs[take[i].job_id].sort);
+    MG_HIP(hipEventRecord(sl.t0, sl.stream));
     std::vector<JobDesc> d;
     tiled_descs(p, (char*)sl.arena, d);
-    MG_TRY(tiled_write_descs(p, (char*)sl.arena, d, s->s_comp));
-    MG_TRY(run_plan(p, args, o, (char*)sl.arena, sl.bins, nullptr, s->s_comp));
-    MG_HIP(hipEventRecord(sl.t1, s->s_comp));
+    MG_TRY(tiled_write_descs(p, (char*)sl.arena, d, sl.stream));
+    MG_TRY(run_plan(p, args, o, (char*)sl.arena, sl.bins, nullptr, sl.stream, n_sorted,
+                    kept.data()));
+    for (size_t i = 0; i < take.size(); i++) {
+        if (!(take[i].flags & MIGRID_SUBMIT_KEEP_SORT) || (take[i].flags & MIGRID_SUBMIT_SAME_UV))
+            continue;
+        Job& j = s->jobs[take[i].job_id];
+        MG_TRY(tiled_keep_sort(p, (const char*)sl.arena, (int)i, j.sort, sl.stream));
+        j.sort_npts = take[i].npts;
+        j.sort_S = take[i].S;
+    }
+    MG_HIP(hipEventRecord(sl.t1, sl.stream));
+    sl.seq = ++s->batch_seq;
+    for (const Request& r : take) {
+        Job& j = s->jobs[r.job_id];
+        j.last_slot = si;
+        j.last_seq = sl.seq;
+    }
+    s->st.sorts_reused += (int64_t)take.size() - n_sorted;
     sl.reqs = std::move(take);
     take.clear();
     s->st.batches++;
     return MIGRID_OK;
 }
 
-// Completes every finished batch at the head of the device queue (mu held).
-// Returns true when at least one finished.
+// Completes every finished batch (mu held).  Returns true when at least one
+// finished.
 bool reap(migrid_server* s)
 {
     bool any = false;
-    while (!s->busy.empty()) {
-        Slot& sl = s->slots[s->busy.front()];
-        const hipError_t q = hipEventQuery(sl.t1);
-        if (q == hipErrorNotReady) break;
+    for (size_t q = 0; q < s->busy.size();) {
+        const int si = s->busy[q];
+        Slot& sl = s->slots[si];
+        const hipError_t e = hipEventQuery(sl.t1);
+        if (e == hipErrorNotReady) {
+            q++;
+            continue;
+        }
         int status = MIGRID_OK;
-        if (q != hipSuccess) {
+        if (e != hipSuccess) {
             status = MIGRID_EDEVICE;
-            snprintf(s->last_err, sizeof s->last_err, "batch failed: %s", hipGetErrorString(q));
+            snprintf(s->last_err, sizeof s->last_err, "batch failed: %s", hipGetErrorString(e));
         } else {
-            float ms = 0.f;
+            float ms = 0.f, a = 0.f, b = 0.f;
             if (hipEventElapsedTime(&ms, sl.t0, sl.t1) == hipSuccess) s->st.gpu_ms += ms;
+            if (hipEventElapsedTime(&a, s->t_ref, sl.t0) == hipSuccess &&
+                hipEventElapsedTime(&b, s->t_ref, sl.t1) == hipSuccess && s->intervals.size() < (1u << 22))
+                s->intervals.emplace_back(a, b);
         }
         for (Request& r : sl.reqs) finish(s, r, status);
         sl.reqs.clear();
-        s->idle.push_back(s->busy.front());
-        s->busy.pop_front();
+        s->idle.push_back(si);
+        s->busy.erase(s->busy.begin() + q);
         prune_done(s);
         any = true;
     }
@@ -389,6 +479,9 @@ void worker_main(migrid_server* s)
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
         if (reap(s)) s->done_cv.notify_all();
+        // requests that admission or batching fails finish here: their waiters
+        // are woken below
+        const int64_t finished = s->st.completed + s->st.failed;
         admit(s);
         bool launched = false;
         while (!s->admitted.empty() && !s->idle.empty()) {
@@ -406,6 +499,7 @@ void worker_main(migrid_server* s)
             s->busy.push_back(si);
             launched = true;
         }
+        if (s->st.completed + s->st.failed != finished) s->done_cv.notify_all();
         if (launched) continue;
         if (s->stop && s->pending.empty() && s->admitted.empty() && s->busy.empty()) break;
         // poll the device queue; submissions wake the worker early
@@ -460,7 +554,7 @@ int migrid_server_create(const migrid_server_cfg* cfg, migrid_server** out)
     if (!s) return fail(MIGRID_ENOMEM, "host allocation of the server failed");
     s->cfg = *cfg;
     if (s->cfg.max_batch == 0) s->cfg.max_batch = 64;
-    if (s->cfg.max_inflight == 0) s->cfg.max_inflight = 2;
+    if (s->cfg.max_inflight == 0) s->cfg.max_inflight = 3;
     if (s->cfg.hbm_budget == 0) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
@@ -470,11 +564,15 @@ int migrid_server_create(const migrid_server_cfg* cfg, migrid_server** out)
         s->cfg.hbm_budget = (int64_t)(fr / 10 * 9);
     }
     memset(&s->st, 0, sizeof s->st);
+    // every stream at the default priority
     hipError_t e = hipStreamCreateWithFlags(&s->s_copy, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s_comp, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&s->t_ref);
+    if (e == hipSuccess) e = hipEventRecord(s->t_ref, s->s_copy);
+    if (e == hipSuccess) e = hipEventSynchronize(s->t_ref);
     s->slots.resize(s->cfg.max_inflight);
     for (int i = 0; i < s->cfg.max_inflight && e == hipSuccess; i++) {
-        e = hipEventCreate(&s->slots[i].t0);
+        e = hipStreamCreateWithFlags(&s->slots[i].stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreate(&s->slots[i].t0);
         if (e == hipSuccess) e = hipEventCreate(&s->slots[i].t1);
         s->idle.push_back(i);
     }
@@ -483,9 +581,10 @@ int migrid_server_create(const migrid_server_cfg* cfg, migrid_server** out)
         for (Slot& sl : s->slots) {
             if (sl.t0) (void)hipEventDestroy(sl.t0);
             if (sl.t1) (void)hipEventDestroy(sl.t1);
+            if (sl.stream) (void)hipStreamDestroy(sl.stream);
         }
+        if (s->t_ref) (void)hipEventDestroy(s->t_ref);
         if (s->s_copy) (void)hipStreamDestroy(s->s_copy);
-        if (s->s_comp) (void)hipStreamDestroy(s->s_comp);
         delete s;
         return rc;
     }
@@ -500,7 +599,8 @@ int migrid_server_submit(migrid_server* s, int64_t job_id, int64_t img_dim, int6
 {
     clear_error();
     if (!s || !ticket) return fail(MIGRID_EINVAL, "NULL argument");
-    if (flags & ~MIGRID_SUBMIT_DEVICE) return fail(MIGRID_EINVAL, "flags=%d", flags);
+    if (flags & ~(MIGRID_SUBMIT_DEVICE | MIGRID_SUBMIT_KEEP_SORT | MIGRID_SUBMIT_SAME_UV))
+        return fail(MIGRID_EINVAL, "flags=%d", flags);
     MG_TRY(validate_shape(npts, img_dim, gcf_dim));
     if (gcf_dim > 64 || img_dim + gcf_dim - 2 >= 65536)
         return fail(MIGRID_ENOTSUP, "served jobs need S <= 64 and N + S - 2 < 65536");
@@ -512,6 +612,7 @@ int migrid_server_submit(migrid_server* s, int64_t job_id, int64_t img_dim, int6
     r.S = gcf_dim;
     r.npts = npts;
     r.host = !(flags & MIGRID_SUBMIT_DEVICE);
+    r.flags = flags;
     if (r.host) {
         // copied now into pinned staging: the caller may reuse its buffers
         const size_t e = s->cfg.in_type == MIGRID_F64 ? 16 : 8;
@@ -615,6 +716,10 @@ int migrid_server_drop_job(migrid_server* s, int64_t job_id)
     if (it == s->jobs.end()) return MIGRID_OK;
     MG_HIP(hipFree(it->second.grid));
     s->hbm_used -= it->second.bytes;
+    if (it->second.sort) {
+        MG_HIP(hipFree(it->second.sort));
+        s->hbm_used -= it->second.sort_bytes;
+    }
     s->jobs.erase(it);
     s->cv.notify_one();
     return MIGRID_OK;
@@ -633,6 +738,25 @@ int migrid_server_get_stats(migrid_server* s, migrid_server_stats* out)
     int64_t running = 0;
     for (int i : s->busy) running += (int64_t)s->slots[i].reqs.size();
     out->running = running;
+    // union of the completed batches' device intervals, and the time two or
+    // more of them overlapped
+    std::vector<std::pair<double, double>> iv = s->intervals;
+    std::sort(iv.begin(), iv.end());
+    double busy = 0, overlap = 0, end = -1e300;
+    for (const auto& x : iv) {
+        if (x.first >= end) {
+            busy += x.second - x.first;
+            end = x.second;
+        } else {
+            overlap += std::min(x.second, end) - x.first;
+            if (x.second > end) {
+                busy += x.second - end;
+                end = x.second;
+            }
+        }
+    }
+    out->gpu_busy_ms = busy;
+    out->overlap_ms = overlap;
     return MIGRID_OK;
 }
 
@@ -646,15 +770,19 @@ int migrid_server_destroy(migrid_server* s)
     }
     s->cv.notify_all();
     if (s->worker.joinable()) s->worker.join();  // drains every submitted request
-    for (auto& kv : s->jobs) (void)hipFree(kv.second.grid);
+    for (auto& kv : s->jobs) {
+        (void)hipFree(kv.second.grid);
+        if (kv.second.sort) (void)hipFree(kv.second.sort);
+    }
     for (Slot& sl : s->slots) {
         if (sl.arena) (void)hipFree(sl.arena);
         if (sl.bins) (void)hipFree(sl.bins);
         (void)hipEventDestroy(sl.t0);
         (void)hipEventDestroy(sl.t1);
+        (void)hipStreamDestroy(sl.stream);
     }
+    (void)hipEventDestroy(s->t_ref);
     (void)hipStreamDestroy(s->s_copy);
-    (void)hipStreamDestroy(s->s_comp);
     delete s;
     return MIGRID_OK;
 }

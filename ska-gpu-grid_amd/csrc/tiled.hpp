@@ -203,7 +203,18 @@ int tiled_write_descs(const TiledPlan& p, char* arena, const std::vector<JobDesc
 // Queue reset, GCF conversion, one bin sort of all jobs, record packing,
 // per-tile counts and work items (one launch each).  For fresh grids the
 // work-item pass zeroes the tiles the owner kernel will not store whole.
-int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_t s);
+//
+// Sorts can be kept between launches (the serving loop's jobs whose
+// visibility coordinates repeat): jobs [n_sorted, n) of the plan are not
+// sorted but take kept[j - n_sorted], a sort saved by tiled_keep_sort from an
+// earlier launch of the same coordinates (n_sorted < 0: sort every job).
+int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_t s,
+                  int n_sorted = -1, const void* const* kept = nullptr);
+// Device bytes of a kept sort of job t (its permutation and CSR offsets).
+size_t tiled_sort_bytes(const TiledJob& t);
+// Saves job j's sort out of the arena into dst (tiled_sort_bytes bytes),
+// numbered within the job, after tiled_prepare on the same stream.
+int tiled_keep_sort(const TiledPlan& p, const char* arena, int j, void* dst, hipStream_t s);
 // The persistent kernels over their queued items: the tap kernel for the
 // small-support jobs, the owner kernel for the rest, then (gather / window)
 // the ordered sum of split tiles.

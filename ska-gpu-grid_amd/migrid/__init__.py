@@ -88,7 +88,9 @@ class ServerCfg(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "submitted", "completed", "failed", "batches", "vis_completed", "hbm_in_use", "hbm_budget",
-        "jobs_resident", "queued", "running")] + [("gpu_ms", ctypes.c_double)]
+        "jobs_resident", "queued", "running")] + [
+        ("gpu_ms", ctypes.c_double), ("gpu_busy_ms", ctypes.c_double),
+        ("overlap_ms", ctypes.c_double), ("sorts_reused", ctypes.c_int64)]
 
 
 class Ceiling(ctypes.Structure):
@@ -558,7 +560,7 @@ def autotune(ctx, jobs, stream=None, **opts):
 
 # ---- serving loop -------------------------------------------------------------
 
-SUBMIT_HOST, SUBMIT_DEVICE = 0, 1
+SUBMIT_HOST, SUBMIT_DEVICE, SUBMIT_KEEP_SORT, SUBMIT_SAME_UV = 0, 1, 2, 4
 
 
 class Server:
@@ -583,7 +585,11 @@ class Server:
         self._keep = {}
         self._N = {}
 
-    def submit(self, job_id, uv, vals, gcf, N, S):
+    def submit(self, job_id, uv, vals, gcf, N, S, keep_sort=False, same_uv=False):
+        """Queues one request.  keep_sort: keep its bin sort resident with the
+        job; same_uv: its uv are those of the job's kept sort (the sort is
+        reused, only the values are packed)."""
+        extra = (SUBMIT_KEEP_SORT if keep_sort else 0) | (SUBMIT_SAME_UV if same_uv else 0)
         ctype = np.complex128 if self.cfg.in_type == F64 else np.complex64
         rtype = np.float64 if self.cfg.in_type == F64 else np.float32
         t = ctypes.c_int64()
@@ -597,14 +603,14 @@ class Server:
             if g.size * g.itemsize != 64 * S * S * 2 * np.dtype(rtype).itemsize:
                 raise MigridError(-1, f"gcf must hold 64*S*S complex values (S={S})")
             check(lib().migrid_server_submit(self.h, job_id, N, S, g.ctypes.data, uv.ctypes.data,
-                                             vals.ctypes.data, uv.shape[0], SUBMIT_HOST,
+                                             vals.ctypes.data, uv.shape[0], SUBMIT_HOST | extra,
                                              ctypes.byref(t)))
         else:
             for x in (uv, vals, gcf):
                 if not x.is_cuda or not x.is_contiguous():
                     raise MigridError(-1, "device tensors must be contiguous and on the GPU")
             check(lib().migrid_server_submit(self.h, job_id, N, S, gcf.data_ptr(), uv.data_ptr(),
-                                             vals.data_ptr(), uv.shape[0], SUBMIT_DEVICE,
+                                             vals.data_ptr(), uv.shape[0], SUBMIT_DEVICE | extra,
                                              ctypes.byref(t)))
             self._keep[t.value] = (uv, vals, gcf)
         self._N[job_id] = N

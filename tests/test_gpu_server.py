@@ -182,3 +182,46 @@ def test_completed_statuses_are_bounded():
             got = srv.read_grid(j)
             count = len(range(j, n, jobs))
             assert np.abs(got - count * one).max() <= 1e-5 * count * np.abs(one).max()
+
+
+@pytest.mark.parametrize("S", [8, 32])
+def test_kept_sort_is_reused_for_repeated_coordinates(S):
+    """A job keeps its request's bin sort; later requests with the same uv and
+    new values reuse it (only their values are packed) and accumulate as if
+    sorted afresh -- also when a batch mixes jobs sorted afresh with jobs
+    taking kept sorts.  A request claiming a kept sort its job does not hold
+    fails alone."""
+    t = torch()
+    N, n = 512, 20000
+    uv, vals, gcf = inputs(1, N, S, n, seed=31)
+    rng = np.random.default_rng(S)
+    v2 = rng.normal(size=vals.shape)
+    uv_b, vals_b, _ = inputs(2, N, S, 7000, seed=32)
+    d = [t.from_numpy(np.ascontiguousarray(x)).cuda() for x in (uv, vals, gcf, v2, 3.0 * v2, uv_b, vals_b)]
+    # a long request first keeps the one slot busy, so the next ones share a batch
+    uv_l, vals_l, gcf_l = inputs(3, 2048, 64, 2_000_000, seed=33)
+    dl = [t.from_numpy(np.ascontiguousarray(x)).cuda() for x in (uv_l, vals_l, gcf_l)]
+    t.cuda.synchronize()
+    with migrid.Server(arith=migrid.ARITH_NATIVE, max_inflight=1) as srv:
+        srv.wait(srv.submit(4, d[0], d[1], d[2], N, S, keep_sort=True))
+        tl = srv.submit(9, *dl, 2048, 64)
+        ts = [srv.submit(4, d[0], d[3], d[2], N, S, same_uv=True),
+              srv.submit(6, d[5], d[6], d[2], N, S)]
+        srv.wait(tl)
+        for tk in ts:
+            srv.wait(tk)
+        srv.wait(srv.submit(4, d[0], d[4], d[2], N, S, same_uv=True))
+        st = srv.stats()
+        assert st["sorts_reused"] == 2 and st["failed"] == 0
+        want = ref(uv, vals, gcf, N, S) + 4.0 * ref(uv, v2, gcf, N, S)
+        assert O.rel_errors(srv.read_grid(4), want)[0] < 1e-12
+        assert O.rel_errors(srv.read_grid(6), ref(uv_b, vals_b, gcf, N, S))[0] < 1e-12
+        # no kept sort for job 7, and job 4's is of 20000 visibilities, not 100
+        with pytest.raises(migrid.MigridError):
+            srv.wait(srv.submit(7, d[0], d[1], d[2], N, S, same_uv=True))
+        with pytest.raises(migrid.MigridError):
+            srv.wait(srv.submit(4, d[0][:100], d[1][:100], d[2], N, S, same_uv=True))
+        srv.wait(srv.submit(4, d[0], d[1], d[2], N, S, same_uv=True))
+        want = want + ref(uv, vals, gcf, N, S)
+        assert O.rel_errors(srv.read_grid(4), want)[0] < 1e-12
+        assert srv.stats()["failed"] == 2
