@@ -81,7 +81,7 @@ def test_make_opts_keeps_every_field():
 
 def test_server_structs_and_defaults():
     assert ctypes.sizeof(migrid.ServerCfg) == 6 * 4 + 8 + ctypes.sizeof(migrid.Opts)
-    assert ctypes.sizeof(migrid.ServerStats) == 11 * 8
+    assert ctypes.sizeof(migrid.ServerStats) == 14 * 8  # 11 int64 + 3 double
     c = migrid.ServerCfg()
     migrid.lib().migrid_server_cfg_default(ctypes.byref(c))
     assert (c.in_type, c.out_type, c.max_batch, c.max_inflight) == (migrid.F64, migrid.F64, 64, 2)
