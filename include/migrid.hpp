@@ -22,7 +22,10 @@
  *
  * Defining MIGRID_DROPIN_GRIDCPU before including this header also defines
  * gridCPU / gridCPU_pz (grid.cu:87-193) with the same argument lists
- * (unpadded output base, planar layout), computed on the GPU.
+ * (unpadded output base, planar layout), computed on the GPU.  With a float2
+ * output, gridCPU_pz writes float planes (N^2 floats each): the reference
+ * casts its output to double* whatever its type (grid.cu:142-144), writing
+ * double planes past the end of an N^2 float2 buffer.
  */
 #ifndef MIGRID_HPP
 #define MIGRID_HPP
