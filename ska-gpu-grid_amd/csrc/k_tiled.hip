@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
     const TI* __restrict__ uv = (const TI*)jb.uv;
     const TI* __restrict__ vals = (const TI*)jb.vals;
     const TI* __restrict__ gcf = (const TI*)jb.gcf;
-    const int S = jb.S, pitch = jb.pitch;
+    const int S = jb.S, pitch = jb.pitch, esize = jb.esize;
     const int64_t p_lo = jb.offs[0], p_hi = jb.offs[jb.nkeys];
     float p1 = 0.f, p2 = 0.f;
     if (VT == kValCompatS || VT == kValCompatI) {
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
         }
         R r;
         r.q = qx | (qy << 16);
-        r.off = qy * (uint32_t)pitch + qx;
+        r.off = (qy * (uint32_t)pitch + qx) * (uint32_t)esize;
         r.vr = (decltype(r.vr))vr;
         r.vi = (decltype(r.vi))vi;
         rec[j] = r;
@@ -402,6 +402,41 @@ constexpr size_t kMaxLds = 160 * 1024 - 4096;
 
 template <typename R>
 using ConstPtr = const __attribute__((address_space(4))) R*;  // scalar-loadable
+
+// Two consecutive records by one scalar load (s_load_dwordx8 for 16-byte
+// records, x16 for 32-byte ones): the byte offset and value of each.
+typedef uint32_t U32x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t U32x16 __attribute__((ext_vector_type(16)));
+template <typename R>
+struct Pair;
+template <>
+struct Pair<RecF32> {
+    static __device__ __forceinline__ void get(ConstPtr<RecF32> rec, int r, uint32_t (&off)[2],
+                                               float (&vr)[2], float (&vi)[2])
+    {
+        const U32x8 w = *(const __attribute__((address_space(4))) U32x8*)(rec + r);
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            off[j] = w[4 * j + 1];
+            vr[j] = __uint_as_float(w[4 * j + 2]);
+            vi[j] = __uint_as_float(w[4 * j + 3]);
+        }
+    }
+};
+template <>
+struct Pair<RecF64> {
+    static __device__ __forceinline__ void get(ConstPtr<RecF64> rec, int r, uint32_t (&off)[2],
+                                               double (&vr)[2], double (&vi)[2])
+    {
+        const U32x16 w = *(const __attribute__((address_space(4))) U32x16*)(rec + r);
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            off[j] = w[8 * j + 1];
+            vr[j] = __hiloint2double((int)w[8 * j + 5], (int)w[8 * j + 4]);
+            vi[j] = __hiloint2double((int)w[8 * j + 7], (int)w[8 * j + 6]);
+        }
+    }
+};
 template <typename R>
 using GlobalPtr = const __attribute__((address_space(1))) R*;
 
@@ -481,6 +516,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
 {
     using P = typename Stage<G>::P;  // product type
     constexpr int NT = 64 * WX * WY;
+    static_assert(((WX * WY) & (WX * WY - 1)) == 0, "a power-of-two number of waves");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     G* sg = (G*)smem_raw;
     __shared__ int s_item;
@@ -488,7 +524,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+    // wave-uniform; the mask (an identity for tid < NT) tells the compiler
+    // w < WX * WY, so the staging loops' row bounds fold away
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6) & (WX * WY - 1);
     const int wx = w % WX, wy = w / WX;
     const int lx = lane & 7, ly = lane >> 3;
 
@@ -589,7 +627,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
             int v = 0;
             const int t = lane >> 1;
             if (wave_on && lane < 2 * NBY && byl + t <= byh) {
-                const int64_t key = ((int64_t)kk * g.nb + byl + t) * g.nb;
+                // 32-bit: keys stay below 2^31 (the plan refuses more)
+                const int key = (kk * g.nb + byl + t) * g.nb;
                 v = offs_v[(lane & 1) ? key + bxh + 1 : key + bxl];
             }
             return v;
@@ -644,29 +683,55 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
                 }
                 __syncthreads();
             }
-            GlobalPtr<G> gk = (GlobalPtr<G>)(uintptr_t)(gtab + (int64_t)k * S * S);
+            if (staged) {
+                // Per lane: the staged element of block (0,0) for q = 0; a
+                // visibility shifts it back by its wave-uniform record offset
+                // (q_y * pitch + q_x elements, stored in bytes), and taps
+                // outside the window read the zero border.  The wave's bin
+                // rows in turn, in one loop: pairs of visibilities from one
+                // scalar record load, then a row's odd last one alone.  (One
+                // loop over the rows keeps the accumulators in the same
+                // registers throughout; a loop per row costs copies in and out
+                // of each.)
+                const char* lb = (const char*)(sg + lane_off + (S - 1 + B + Yw) * pitch +
+                                               (S - 1 + B + Xw));
+                auto taps = [&](uint32_t off, P (&gr)[2][2], P (&gi)[2][2]) {
+                    const G* b = (const G*)(lb - off);
 #pragma unroll
-            for (int t = 0; t < NBY; t++) {
-                const int r0 = __builtin_amdgcn_readlane(bnd, 2 * t);
-                const int r1 = __builtin_amdgcn_readlane(bnd, 2 * t + 1);
-                if (r0 >= r1) continue;
-                if (staged) {
-                    // per lane: the staged element of block (0,0) for q = 0;
-                    // a visibility shifts it by its wave-uniform record offset
-                    // q_y * pitch + q_x (taps outside the window read zeros)
-                    const G* lb = sg + lane_off + (S - 1 + B + Yw) * pitch + (S - 1 + B + Xw);
-#pragma unroll 2
-                    for (int r = r0; r < r1; r++) {
-                        const G* b = lb - (int)rec[r].off;
-                        P gr[2][2], gi[2][2];
+                    for (int i = 0; i < 2; i++)
 #pragma unroll
-                        for (int i = 0; i < 2; i++)
-#pragma unroll
-                            for (int j = 0; j < 2; j++)
-                                Stage<G>::get(b[8 * i * pitch + 8 * j], gr[i][j], gi[i][j]);
-                        mac4<P, TA>(acc, (P)rec[r].vr, (P)rec[r].vi, gr, gi);
+                        for (int j = 0; j < 2; j++)
+                            Stage<G>::get(b[8 * i * pitch + 8 * j], gr[i][j], gi[i][j]);
+                };
+                int t = 0;
+                int r = __builtin_amdgcn_readlane(bnd, 0), e = __builtin_amdgcn_readlane(bnd, 1);
+                for (;;) {
+                    for (; r + 1 < e; r += 2) {
+                        uint32_t off[2];
+                        decltype(+rec[0].vr) vr[2], vi[2];
+                        Pair<R>::get(rec, r, off, vr, vi);
+                        P gr0[2][2], gi0[2][2], gr1[2][2], gi1[2][2];
+                        taps(off[0], gr0, gi0);
+                        taps(off[1], gr1, gi1);
+                        mac4<P, TA>(acc, (P)vr[0], (P)vi[0], gr0, gi0);
+                        mac4<P, TA>(acc, (P)vr[1], (P)vi[1], gr1, gi1);
                     }
-                } else {
+                    if (r < e) {
+                        P gr0[2][2], gi0[2][2];
+                        taps(rec[r].off, gr0, gi0);
+                        mac4<P, TA>(acc, (P)rec[r].vr, (P)rec[r].vi, gr0, gi0);
+                    }
+                    if (++t == NBY) break;
+                    r = __builtin_amdgcn_readlane(bnd, 2 * t);
+                    e = __builtin_amdgcn_readlane(bnd, 2 * t + 1);
+                }
+            } else {
+                // a sparse sub-kernel read from the table in memory
+                GlobalPtr<G> gk = (GlobalPtr<G>)(uintptr_t)(gtab + (int64_t)k * S * S);
+#pragma unroll
+                for (int t = 0; t < NBY; t++) {
+                    const int r0 = __builtin_amdgcn_readlane(bnd, 2 * t);
+                    const int r1 = __builtin_amdgcn_readlane(bnd, 2 * t + 1);
                     for (int r = r0; r < r1; r++) {
                         const uint32_t q = rec[r].q;
                         const int c0 = S - 1 - ((int)(q & 0xffffu) - Xw);
@@ -1451,6 +1516,7 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
         pj[j].nkeys = t.bg.nkeys;
         pj[j].S = t.a.S;
         pj[j].pitch = stage_pitch(t.a.S);
+        pj[j].esize = p.key.stage == kStageF64 ? 16 : p.key.stage == kStageF32 ? 8 : 4;
         if (t.convert) {
             ConvJob c;
             c.src = t.a.gcf;

@@ -15,7 +15,7 @@ namespace migrid {
 // q = floor(coordinate) + S/2 (so every binned visibility has 0 <= q < 2^16).
 struct alignas(16) RecF32 {
     uint32_t q;
-    uint32_t off;  // q_y * pitch + q_x of the staged-kernel layout
+    uint32_t off;  // (q_y * pitch + q_x) x staged element bytes, in the staged-kernel layout
     float vr, vi;
 };
 struct alignas(16) RecF64 {
@@ -134,6 +134,7 @@ struct PackJob {
     int64_t e_base;
     int64_t nkeys;
     int S, pitch;
+    int esize;  // bytes of one staged element: record offsets are in bytes
 };
 
 // GCF table conversion to the staged type (device table).
