@@ -86,7 +86,9 @@ typedef struct migrid_opts {
     int32_t accumulate;     /* 0: fresh grid (zeroed first), 1: add into it */
     int32_t debug_coverage; /* 1: every tap adds (1, n) -- the reference's
                                DEBUG1 coverage map (grid.cu:127-129) */
-    int32_t tile;           /* TILED/GATHER/WINDOW tile edge in pixels, 0 = auto */
+    int32_t tile;           /* TILED/GATHER/WINDOW tile: 16 or 32 (square tiles of
+                               16 x 16 wave blocks), 64 (128 x 64 tiles of 8 x 64
+                               wave blocks), 0 = auto (64) */
     int32_t vis_per_wave;   /* TILED work-item size, 0 = auto */
     double gcf_sigma;       /* ANALYTIC: Gaussian sigma in pixels (0: S/6) */
     double gcf_beta;        /* ANALYTIC: quadratic phase coefficient */

@@ -41,6 +41,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <utility>
 
 #include "bin.hpp"
 #include "device_common.hpp"
@@ -187,10 +188,10 @@ __global__ __launch_bounds__(128) void k_tile_counts(const JobDesc* __restrict__
     const TileGeom g = d.g;
     const int32_t* __restrict__ offs = d.offs;
     int32_t* __restrict__ tcount = (int32_t*)d.tcount;
-    for (int64_t b = blockIdx.x; b < (int64_t)g.ntx * kSubKernels; b += gridDim.x) {
+    for (int64_t b = blockIdx.x; b < (int64_t)g.nty * kSubKernels; b += gridDim.x) {
         const int ty = (int)(b / kSubKernels), k = (int)(b % kSubKernels);
-        const int byl = ty * g.T / g.H;
-        const int byh = min(g.nb - 1, (ty * g.T + g.T + g.S - 2) / g.H);
+        const int byl = ty * g.Ty / g.H;
+        const int byh = min(g.nb - 1, (ty * g.Ty + g.Ty + g.S - 2) / g.H);
         for (int tx = threadIdx.x; tx < g.ntx; tx += blockDim.x) {
             const int bxl = tx * g.T / g.H;
             const int bxh = min(g.nb - 1, (tx * g.T + g.T + g.S - 2) / g.H);
@@ -206,11 +207,11 @@ __global__ __launch_bounds__(128) void k_tile_counts(const JobDesc* __restrict__
 
 // Zeroes the pixels of one tile (the whole wave; lanes along a row).
 template <typename TO>
-__device__ void zero_tile(TO* grid, int N, int X0, int Y0, int T, bool planar)
+__device__ void zero_tile(TO* grid, int N, int X0, int Y0, int T, int Ty, bool planar)
 {
     const int lane = threadIdx.x & 63;
     const int64_t NN = (int64_t)N * N;
-    const int xe = min(N, X0 + T), ye = min(N, Y0 + T);
+    const int xe = min(N, X0 + T), ye = min(N, Y0 + Ty);
     for (int y = Y0; y < ye; y++)
         for (int x = X0 + lane; x < xe; x += 64) {
             const int64_t p = (int64_t)y * N + x;
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs,
     const bool det = jobs[job].det != 0;
     const int lane = threadIdx.x & 63;
     const int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int64_t ntiles = (int64_t)g.ntx * g.ntx;
+    const int64_t ntiles = (int64_t)g.ntx * g.nty;
     if (tile >= ntiles) return;
     const int tx = (int)(tile % g.ntx), ty = (int)(tile / g.ntx);
     const int64_t cnt = tcount[tile * kSubKernels + lane];
@@ -265,9 +266,9 @@ __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs,
     const int64_t total = __shfl(inc, 63, 64);
     if (total == 0) {
         if (fresh) {
-            zero_tile((TO*)jobs[job].grid, g.N, tx * g.T, ty * g.T, g.T, planar != 0);
+            zero_tile((TO*)jobs[job].grid, g.N, tx * g.T, ty * g.Ty, g.T, g.Ty, planar != 0);
             if (jobs[job].grid_lo)
-                zero_tile((TO*)jobs[job].grid_lo, g.N, tx * g.T, ty * g.T, g.T, planar != 0);
+                zero_tile((TO*)jobs[job].grid_lo, g.N, tx * g.T, ty * g.Ty, g.T, g.Ty, planar != 0);
         }
         return;
     }
@@ -326,9 +327,9 @@ __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs,
     }
     if (lane == 0) split[tile] = n_items > 1 || shared;
     if (fresh && n_items > 1 && pbase < 0) {  // split tiles flushed with atomics start at zero
-        zero_tile((TO*)jobs[job].grid, g.N, tx * g.T, ty * g.T, g.T, planar != 0);
+        zero_tile((TO*)jobs[job].grid, g.N, tx * g.T, ty * g.Ty, g.T, g.Ty, planar != 0);
         if (jobs[job].grid_lo)
-            zero_tile((TO*)jobs[job].grid_lo, g.N, tx * g.T, ty * g.T, g.T, planar != 0);
+            zero_tile((TO*)jobs[job].grid_lo, g.N, tx * g.T, ty * g.Ty, g.T, g.Ty, planar != 0);
     }
 }
 
@@ -437,6 +438,24 @@ struct Pair<RecF64> {
         }
     }
 };
+// The same from a byte offset into the records (32-bit: the scalar load
+// takes it as its offset register), and one record alone.
+template <typename R>
+__device__ __forceinline__ void pair_at(ConstPtr<char> base, uint32_t o, uint32_t (&off)[2],
+                                        decltype(+R::vr) (&vr)[2], decltype(+R::vr) (&vi)[2])
+{
+    Pair<R>::get((ConstPtr<R>)(base + o), 0, off, vr, vi);
+}
+template <typename R>
+__device__ __forceinline__ void rec_at(ConstPtr<char> base, uint32_t o, uint32_t& off,
+                                       decltype(+R::vr)& vr, decltype(+R::vr)& vi)
+{
+    ConstPtr<R> r = (ConstPtr<R>)(base + o);
+    off = r->off;
+    vr = r->vr;
+    vi = r->vi;
+}
+
 template <typename R>
 using GlobalPtr = const __attribute__((address_space(1))) R*;
 
@@ -788,6 +807,357 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner(
     }
 }
 
+// ---- the column-block owner kernel (128 x 64 tiles) ----------------------------
+// A workgroup of 16 waves owns a 128 x 64 tile as 16 blocks of 8 x 64 pixels:
+// wave w's block starts at (X0 + 8 w, Y0), and lane (lx, ly) = (lane & 7,
+// lane >> 3) owns the eight pixels (Xw + lx, Yw + ly + 8 i), i = 0..7, of its
+// column, one in each row group of 8.  Bins are 8 x 8 pixels of q, so a wave
+// lists col_ncols(S) bins in each of its col_nrows(S) bin rows, and every
+// visibility of bin row t reaches the same row groups [max(0, t - D),
+// min(7, t)], D = col_depth(S): a lane reads and multiplies only those.  Per
+// axis an S-wide window then costs about S + 7 executed columns and rows,
+// against S + 15 for 16 x 16 blocks (at S = 64, 81% of the executed taps are
+// real rather than 66%); a visibility's taps need one address per lane, the
+// groups being immediate offsets apart; and a 64-row block meets a window in
+// two blocks' worth of rows at most, so each visibility is visited by about
+// 18 waves rather than 25.
+//
+// Staged layout: sub-kernel element (r, c) sits at LDS element
+// (r + L) P + c + L, with L = col_lead(S) zero rows and columns before the
+// kernel and 7 after it (a row's trailing zeros are the next row's leading
+// ones, so P >= S + L).  A lane reads row r = ly + 8 i - dy + S - 1, with
+// dy = q_y - Yw in [8t, 8t + 7] and i one of bin row t's groups, and column
+// c = lx - dx + S - 1, with dx = q_x - Xw in [0, 8 col_ncols(S) - 1]: both lie
+// in [-L, S + 6], so every read lands in the kernel or in zeros.
+constexpr int kColGroups = 8;  // row groups per lane
+constexpr int kColTx = 128, kColTy = 64;  // tile width and height
+constexpr int kColH = 8;                  // bin edge of the column kernel's jobs
+__host__ __device__ constexpr int col_depth(int S) { return (S + 6) / 8; }
+__host__ __device__ constexpr int col_lead(int S) { return 8 * col_depth(S) + 8 - S; }
+// S + L is a multiple of 8; the pitch is 8 or 24 mod 32 elements, so the 8
+// rows of an 8 x 8 lane block fall in different LDS banks, and a multiple of
+// 8, so row groups (8 P elements apart) pair up in ds_read2st64 reads.
+__host__ __device__ constexpr int col_pitch(int S)
+{
+    return (S + col_lead(S)) % 16 == 8 ? S + col_lead(S) : S + col_lead(S) + 8;
+}
+__host__ __device__ constexpr int col_rows(int S) { return S + col_lead(S) + 8; }
+__host__ __device__ constexpr int col_ncols(int S) { return col_depth(S) + 1; }
+__host__ __device__ constexpr int col_nrows(int S) { return (8 * kColGroups + S - 2) / 8 + 1; }
+
+template <int... I, typename F>
+__device__ __forceinline__ void static_for_seq(std::integer_sequence<int, I...>, F&& f)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    static_for_seq(std::make_integer_sequence<int, N>{}, f);
+}
+
+// SC: every job's support as a compile-time constant (0: read per job), which
+// makes the bin rows' group ranges and the LDS offsets constants.
+template <typename G, typename TA, typename R, typename TI, typename TO, bool PLANAR, int GM,
+          int SC>
+__global__ __launch_bounds__(1024)
+__attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner_col(
+    const JobDesc* __restrict__ jobs, const Item* __restrict__ items,
+    QueueState* __restrict__ qs, int smax, int stage_min)
+{
+    using P = typename Stage<G>::P;  // product type
+    using V = decltype(+((const R*)nullptr)->vr);  // record value type
+    constexpr int NW = 16, NT = 64 * NW, NG = kColGroups;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    G* sg = (G*)smem_raw;
+    __shared__ int s_item;
+    __shared__ Cplx<double> s_fac[2][64];  // analytic separable factors
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6) & (NW - 1);
+    const int lx = lane & 7, ly = lane >> 3;
+
+    for (int e = tid; e < col_rows(smax) * col_pitch(smax); e += NT)
+        sg[e] = Stage<G>::make(0.0, 0.0);
+    int staged_s = -1;  // support of the layout now in LDS
+
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(&qs->head, 1);
+        __syncthreads();
+        const int it_idx = s_item;
+        if (it_idx >= qs->nitems) break;
+        const Item it = items[it_idx];
+        const int tile = __builtin_amdgcn_readfirstlane(it.tile);
+        const int klo = __builtin_amdgcn_readfirstlane(it.klo);
+        const int khi = __builtin_amdgcn_readfirstlane(it.khi);
+        const int job = __builtin_amdgcn_readfirstlane(it.job);
+        const int pslot = __builtin_amdgcn_readfirstlane(it.pslot);
+        const TileGeom g = jobs[job].g;
+        ConstPtr<R> rec = (ConstPtr<R>)(uintptr_t)jobs[job].rec;
+        ConstPtr<int32_t> offs = (ConstPtr<int32_t>)(uintptr_t)jobs[job].offs;
+        const G* __restrict__ gtab = (const G*)jobs[job].gcf;  // table in the staged type
+        TO* __restrict__ grid = (TO*)jobs[job].grid;
+        TO* __restrict__ grid_lo = (TO*)jobs[job].grid_lo;
+        const double sigma = jobs[job].sigma, beta = jobs[job].beta;
+        const bool atomic = jobs[job].split[tile] != 0;
+        const bool fresh = jobs[job].fresh != 0;
+        const int S = SC ? SC : g.S;
+        const int L = col_lead(S);
+        const int pitch = col_pitch(S);
+        const int64_t NN = (int64_t)g.N * g.N;
+        if (S != staged_s) {
+            // a new layout: clear the old interior, then (constant-tap form)
+            // write the all-ones kernel once
+            __syncthreads();
+            if (staged_s > 0) {
+                const int ol = col_lead(staged_s), op = col_pitch(staged_s);
+                for (int e = tid; e < staged_s * staged_s; e += NT)
+                    sg[(e / staged_s + ol) * op + e % staged_s + ol] = Stage<G>::make(0.0, 0.0);
+                __syncthreads();
+            }
+            if (GM == 1)
+                for (int e = tid; e < S * S; e += NT)
+                    sg[(e / S + L) * pitch + e % S + L] = Stage<G>::make(1.0, 0.0);
+            staged_s = S;
+        }
+        __syncthreads();
+        const int tx = tile % g.ntx, ty = tile / g.ntx;
+        const int X0 = tx * kColTx, Y0 = ty * kColTy;
+        const int Xw = X0 + 8 * w, Yw = Y0;
+        const int nb = g.nb;
+        // this wave's bins: columns bxl .. bxl + ncx - 1, rows byl .. byl + nry - 1
+        // (fewer at the grid's far edge; none for a block past it, whose wave
+        // still takes part in the workgroup's barriers)
+        const int bxl = Xw / kColH, byl = Yw / kColH;
+        const bool on = Xw < g.N && Yw < g.N;
+        const int ncx = on ? min(nb - bxl, col_ncols(S)) : 0;
+        const int nry = on && ncx > 0 ? min(nb - byl, col_nrows(S)) : 0;
+        const int D = col_depth(S);
+
+        Cplx<TA> acc[NG];
+#pragma unroll
+        for (int i = 0; i < NG; i++) acc[i].re = acc[i].im = (TA)0;
+
+        // lane k counts the workgroup's visibilities for sub-kernel k; the
+        // ballots give the non-empty and the staged sub-kernels of this item
+        uint64_t nz_mask, st_mask;
+        {
+            const int cnt = (lane >= klo && lane <= khi)
+                                ? jobs[job].tcount[(int64_t)tile * kSubKernels + lane]
+                                : 0;
+            nz_mask = __ballot(cnt > 0);
+            st_mask = (GM != 0 || SC != 0) ? nz_mask : __ballot(cnt >= stage_min && cnt > 0);
+        }
+        // sub-kernel rows this wave stages: w, w + NW, ... (one column per
+        // lane); the next staged sub-kernel is fetched into registers while
+        // the current one is in use
+        constexpr bool PF = GM == 0;
+        constexpr int PR = PF ? (64 + NW - 1) / NW : 1;
+        G pf[PR];
+        auto fetch_rows = [&](int kk) {
+            GlobalPtr<G> src = (GlobalPtr<G>)(uintptr_t)(gtab + (int64_t)kk * S * S);
+#pragma unroll
+            for (int p = 0; p < PR; p++) {
+                const int r = w + p * NW;
+                if (r < S && lane < S) pf[p] = gload<G>(src, r * S + lane);
+            }
+        };
+        if (PF && st_mask) fetch_rows(__builtin_ctzll(st_mask));
+
+        // Per lane: the staged element of group 0 for q = 0, in bytes; a
+        // visibility shifts it back by its record offset (q_y P + q_x
+        // elements, in bytes), group i lies 8 i P elements further.
+        const char* lb = (const char*)sg +
+                         (size_t)((Yw + ly + S - 1 + L) * pitch + Xw + lx + S - 1 + L) * sizeof(G);
+
+        // CSR bounds of the wave's bin rows for one sub-kernel, gathered by
+        // one vector load (lane 2t: start of row t, lane 2t + 1: its end) and
+        // fetched one sub-kernel ahead; row t's key is key0 + t nb
+        GlobalPtr<int32_t> offs_v = (GlobalPtr<int32_t>)(uintptr_t)jobs[job].offs;
+        auto load_bounds = [&](int kk) -> int {
+            int v = 0;
+            if (lane < 2 * nry) {
+                const int key = (kk * nb + byl + (lane >> 1)) * nb + bxl;
+                v = offs_v[(lane & 1) ? key + ncx : key];
+            }
+            return v;
+        };
+        int bnd_next = nz_mask ? load_bounds(__builtin_ctzll(nz_mask)) : 0;
+
+        for (uint64_t todo = nz_mask; todo; todo &= todo - 1) {
+            const int k = __builtin_ctzll(todo);
+            const bool staged = (st_mask >> k) & 1;
+            // first key of the wave's bin rows: row t at key0 + t nb
+            const int key0 = (k * nb + byl) * nb + bxl;
+            const int bnd = bnd_next;
+            {
+                const uint64_t rest = todo & (todo - 1);
+                if (rest) bnd_next = load_bounds(__builtin_ctzll(rest));
+            }
+            if (staged && GM != 1) {
+                __syncthreads();  // previous sub-kernel no longer read
+                if (PF) {
+#pragma unroll
+                    for (int p = 0; p < PR; p++) {
+                        const int r = w + p * NW;
+                        if (r < S && lane < S) sg[(r + L) * pitch + lane + L] = pf[p];
+                    }
+                    const uint64_t later = st_mask & ~((2ull << k) - 1);
+                    if (later) fetch_rows(__builtin_ctzll(later));
+                } else {
+                    // separable analytic GCF: G[r][c] = f(dy_r) * f(dx_c),
+                    // f(d) = exp(-d^2/(2 sigma^2)) * exp(i beta d^2)
+                    const int sx = k % kOversample, sy = k / kOversample;
+                    for (int e = tid; e < 2 * S; e += NT) {
+                        const int axis = e / S, t = e % S;
+                        const double d = (double)(t - S / 2 + 1) - (axis ? sy : sx) / 8.0;
+                        const double q = d * d;
+                        const double amp = exp(-q / (2.0 * sigma * sigma));
+                        double sn, cs;
+                        sincos(beta * q, &sn, &cs);
+                        s_fac[axis][t].re = amp * cs;
+                        s_fac[axis][t].im = amp * sn;
+                    }
+                    __syncthreads();
+                    for (int e = tid; e < S * S; e += NT) {
+                        const int r = e / S, c = e % S;
+                        const Cplx<double> a = s_fac[1][r], b = s_fac[0][c];
+                        sg[(r + L) * pitch + c + L] =
+                            Stage<G>::make(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+                    }
+                }
+                __syncthreads();
+            }
+            if (staged) {
+                if constexpr (SC != 0) {
+                    // the support a constant: each bin row's groups and the
+                    // group offsets are constants
+                    constexpr int NR = col_nrows(SC), DC = col_depth(SC), PC = col_pitch(SC);
+                    constexpr uint32_t RS = sizeof(R);
+                    ConstPtr<char> rbase = (ConstPtr<char>)rec;
+                    // the bounds as byte offsets into the records, and a mask
+                    // of the non-empty rows (bit 2t: row t), so an empty row
+                    // costs one bit test
+                    const uint32_t bb = (uint32_t)bnd * RS;
+                    const uint32_t bb_next =
+                        (uint32_t)__builtin_amdgcn_mov_dpp((int)bb, 0xB1, 0xF, 0xF, false);
+                    const uint64_t rows_on = __ballot(!(lane & 1) && bb < bb_next);
+                    static_for<NR>([&](auto tc) {
+                        constexpr int t = decltype(tc)::value;
+                        constexpr int A = t - DC > 0 ? t - DC : 0;
+                        constexpr int B = t < NG - 1 ? t : NG - 1;
+                        if (!((rows_on >> (2 * t)) & 1)) return;
+                        uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)bb, 2 * t);
+                        const uint32_t oe = (uint32_t)__builtin_amdgcn_readlane((int)bb, 2 * t + 1);
+                        // the taps of one visibility
+                        auto one = [&](uint32_t off, V vr, V vi) {
+                            const G* b = (const G*)(lb - off);
+                            P gr[NG], gi[NG];
+#pragma unroll
+                            for (int i = A; i <= B; i++) Stage<G>::get(b[8 * i * PC], gr[i], gi[i]);
+#pragma unroll
+                            for (int i = A; i <= B; i++)
+                                mac1<P, TA>(acc[i], (P)vr, (P)vi, gr[i], gi[i]);
+                        };
+                        // pairs of visibilities from one scalar record load,
+                        // then the row's odd last one alone
+                        const uint32_t oe1 = oe - RS;
+                        for (; o < oe1; o += 2 * RS) {
+                            uint32_t off[2];
+                            V vr[2], vi[2];
+                            pair_at<R>(rbase, o, off, vr, vi);
+                            one(off[0], vr[0], vi[0]);
+                            // rows of many groups: the second visibility's
+                            // taps are read after the first one's products,
+                            // so one visibility's taps are live at a time
+                            if constexpr (B - A + 1 > 4) __builtin_amdgcn_sched_barrier(0);
+                            one(off[1], vr[1], vi[1]);
+                        }
+                        if (o == oe1) {
+                            uint32_t off;
+                            V vr, vi;
+                            rec_at<R>(rbase, o, off, vr, vi);
+                            one(off, vr, vi);
+                        }
+                    });
+                } else {
+                    // any support: the groups of bin row t are tested per
+                    // visibility (wave-uniform branches)
+                    for (int t = 0; t < nry; t++) {
+                        const int A = max(0, t - D), B = min(NG - 1, t);
+                        const int r0 = offs[key0 + t * nb], r1 = offs[key0 + t * nb + ncx];
+                        for (int r = r0; r < r1; r++) {
+                            const G* b = (const G*)(lb - rec[r].off);
+                            const P vr = (P)rec[r].vr, vi = (P)rec[r].vi;
+#pragma unroll
+                            for (int i = 0; i < NG; i++) {
+                                if (i < A || i > B) continue;
+                                P gr, gi;
+                                Stage<G>::get(b[8 * i * pitch], gr, gi);
+                                mac1<P, TA>(acc[i], vr, vi, gr, gi);
+                            }
+                        }
+                    }
+                }
+            } else if constexpr (SC == 0) {
+                // a sparse sub-kernel read from the table in memory (any
+                // support; with the support a constant every sub-kernel is
+                // staged, and the registers this path would hold go to the
+                // staged loop)
+                GlobalPtr<G> gk = (GlobalPtr<G>)(uintptr_t)(gtab + (int64_t)k * S * S);
+                for (int t = 0; t < nry; t++) {
+                    const int A = max(0, t - D), B = min(NG - 1, t);
+                    const int r0 = offs[key0 + t * nb], r1 = offs[key0 + t * nb + ncx];
+                    for (int r = r0; r < r1; r++) {
+                        const uint32_t q = rec[r].q;
+                        const int c = S - 1 - ((int)(q & 0xffffu) - Xw) + lx;
+                        const int rr0 = S - 1 - ((int)(q >> 16) - Yw) + ly;
+                        const P vr = (P)rec[r].vr, vi = (P)rec[r].vi;
+                        // groups in fours, 32-bit element offsets: few taps
+                        // and addresses live at a time
+#pragma unroll
+                        for (int i = 0; i < NG; i++) {
+                            if (i < A || i > B) continue;
+                            const int rr = rr0 + 8 * i;
+                            P gr = (P)0, gi = (P)0;
+                            if ((unsigned)c < (unsigned)S && (unsigned)rr < (unsigned)S)
+                                Stage<G>::get(gload<G>(gk, rr * S + c), gr, gi);
+                            mac1<P, TA>(acc[i], vr, vi, gr, gi);
+                            if (i % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+                        }
+                    }
+                }
+            }
+        }
+        // flush: a split tile of the ordered-reduction mode writes its
+        // partial tile (128 x 64 accumulators) to its slot
+        if (pslot >= 0) {
+            TA* part = (TA*)jobs[job].partials + (int64_t)pslot * kColTx * kColTy * 2;
+#pragma unroll
+            for (int i = 0; i < NG; i++) {
+                const int e = (Yw - Y0 + ly + 8 * i) * kColTx + (Xw - X0 + lx);
+                part[2 * e] = acc[i].re;
+                part[2 * e + 1] = acc[i].im;
+            }
+            __syncthreads();  // s_item reuse
+            continue;
+        }
+        const int x = Xw + lx;
+#pragma unroll
+        for (int i = 0; i < NG; i++) {
+            const int y = Yw + ly + 8 * i;
+            if (x >= g.N || y >= g.N) continue;
+            const int64_t p = (int64_t)y * g.N + x;
+            if (PLANAR)
+                flush_px(grid, grid_lo, p, NN + p, acc[i].re, acc[i].im, atomic, fresh);
+            else
+                flush_px(grid, grid_lo, 2 * p, 2 * p + 1, acc[i].re, acc[i].im, atomic, fresh);
+        }
+        __syncthreads();  // s_item reuse
+    }
+}
+
 // ---- small supports: one tap per lane --------------------------------------------
 // At S <= 16 a wave's 16 x 16 block is mostly outside a visibility's S x S
 // window (at S = 8 about one executed tap in eight is real), and the
@@ -1007,14 +1377,14 @@ __global__ __launch_bounds__(256) void k_reduce_tiles(const JobDesc* __restrict_
         const JobDesc& d = jobs[ri.job];
         const TileGeom g = d.g;
         const int tx = ri.tile % g.ntx, ty = ri.tile / g.ntx;
-        const int64_t slot = (int64_t)g.T * g.T * 2;
+        const int64_t slot = (int64_t)g.T * g.Ty * 2;
         const TA* __restrict__ part = (const TA*)d.partials + (int64_t)ri.pslot * slot;
         TO* grid = (TO*)d.grid;
         TO* lo = (TO*)d.grid_lo;
         const bool fresh = d.fresh != 0;
         const int64_t NN = (int64_t)g.N * g.N;
-        for (int e = threadIdx.x; e < g.T * g.T; e += blockDim.x) {
-            const int x = tx * g.T + e % g.T, y = ty * g.T + e / g.T;
+        for (int e = threadIdx.x; e < g.T * g.Ty; e += blockDim.x) {
+            const int x = tx * g.T + e % g.T, y = ty * g.Ty + e / g.T;
             if (x >= g.N || y >= g.N) continue;
             TA re = 0, im = 0;
             for (int i = 0; i < ri.n; i++) {
@@ -1097,15 +1467,49 @@ int run_owner(const OwnerLaunch& L, hipStream_t s)
     return MIGRID_OK;
 }
 
-// Tile shapes: a workgroup of (T/16)^2 waves, each owning a 16 x 16 block.
+template <typename G, typename TA, typename R, typename TI, typename TO, bool PLANAR, int GM,
+          int SC = 0>
+int run_owner_col(const OwnerLaunch& L, hipStream_t s)
+{
+    auto kern = k_owner_col<G, TA, R, TI, TO, PLANAR, GM, SC>;
+    if (L.lds > kMaxLds)
+        return fail(MIGRID_ENOTSUP, "tiled: the staged sub-kernel needs %zu bytes of LDS (at most %zu)",
+                    L.lds, kMaxLds);
+    MG_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)L.lds));
+    int per_cu = 0;
+    MG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 1024, L.lds));
+    if (per_cu < 1) return fail(MIGRID_ENOTSUP, "tiled: kernel does not fit (lds=%zu)", L.lds);
+    int dev = 0, ncu = 0;
+    MG_HIP(hipGetDevice(&dev));
+    MG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (getenv("MIGRID_VERBOSE")) {
+        hipFuncAttributes fa;
+        MG_HIP(hipFuncGetAttributes(&fa, (const void*)kern));
+        fprintf(stderr,
+                "[migrid] column owner launch: %d CUs x %d WG/CU (occupancy %d), lds dyn %zu + "
+                "static %zu, vgpr %d, spill %zu\n",
+                ncu, L.wg_per_cu > 0 ? L.wg_per_cu : per_cu, per_cu, L.lds,
+                (size_t)fa.sharedSizeBytes, fa.numRegs, (size_t)fa.localSizeBytes);
+    }
+    if (L.wg_per_cu > 0) per_cu = L.wg_per_cu;
+    hipLaunchKernelGGL(kern, dim3(ncu * per_cu), dim3(1024), L.lds, s, L.jobs, L.items, L.qs,
+                       L.smax, L.stage_min > 0 ? L.stage_min : kStageMinDefault);
+    MG_HIP(hipGetLastError());
+    return MIGRID_OK;
+}
+
+// Tile shapes: the tile option 64 runs the column-block kernel (128 x 64
+// tiles, 16 waves of 8 x 64 pixels); 32 and 16 a workgroup of (T/16)^2 waves,
+// each owning a 16 x 16 block.
 template <typename G, typename TA, typename R, typename TI, typename TO, bool PLANAR, int GM>
 int run_shape(int T, const OwnerLaunch& L, hipStream_t s)
 {
     if (T == 64) {
         // every job at S = 64 or at S = 32: the support as a constant
-        if (L.sc == 64) return run_owner<4, 4, G, TA, R, TI, TO, PLANAR, GM, 64>(L, s);
-        if (L.sc == 32) return run_owner<4, 4, G, TA, R, TI, TO, PLANAR, GM, 32>(L, s);
-        return run_owner<4, 4, G, TA, R, TI, TO, PLANAR, GM>(L, s);
+        if (L.sc == 64) return run_owner_col<G, TA, R, TI, TO, PLANAR, GM, 64>(L, s);
+        if (L.sc == 32) return run_owner_col<G, TA, R, TI, TO, PLANAR, GM, 32>(L, s);
+        return run_owner_col<G, TA, R, TI, TO, PLANAR, GM>(L, s);
     }
     if (T == 32) return run_owner<2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);
     if (T == 16) return run_owner<1, 1, G, TA, R, TI, TO, PLANAR, GM>(L, s);
@@ -1234,11 +1638,18 @@ int dispatch_taps(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
     }
 }
 
-TileCfg pick_tile(int S, int tile_opt)
+TileCfg pick_tile(int S, int tile_opt, bool taps)
 {
     TileCfg c;
     c.T = tile_opt > 0 ? tile_opt : 64;
-    c.H = 16;  // bins match the 16 x 16 block a wave owns
+    c.Ty = c.T;
+    c.H = 16;  // the 16 x 16 block of a wave (tiles of 16, 32), the tap kernel
+    if (c.T == 64 && !taps) {
+        // the column kernel: 128 x 64 tiles of 8 x 64 wave blocks, bins of 8
+        c.T = kColTx;
+        c.Ty = kColTy;
+        c.H = kColH;
+    }
     return c;
 }
 
@@ -1277,8 +1688,11 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         k.gm = (x.debug || x.gcf_mode == MIGRID_GCF_COMPAT) ? 1
                : x.gcf_mode == MIGRID_GCF_ANALYTIC        ? 2
                                                           : 0;
-        const TileCfg tc = pick_tile(x.S, o.tile);
-        k.T = tc.T;
+        const int t_edge = o.tile > 0 ? o.tile : 64;
+        const bool taps = o.strategy == MIGRID_TILED && x.S <= tap_smax &&
+                          (t_edge + x.S - 2) / 16 + 1 <= kTapRows;
+        const TileCfg tc = pick_tile(x.S, o.tile, taps);
+        k.T = t_edge;  // the tile option; each job's geometry is in t.g
         int vt = kValPlain;
         if (x.debug)
             vt = kValDebug;
@@ -1297,9 +1711,11 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         t.g.N = x.N;
         t.g.S = x.S;
         t.g.T = tc.T;
+        t.g.Ty = tc.Ty;
         t.g.H = tc.H;
         t.g.nb = t.bg.nb;
         t.g.ntx = (x.N + tc.T - 1) / tc.T;
+        t.g.nty = (x.N + tc.Ty - 1) / tc.Ty;
         t.rec64 = k.stage == kStageF64;
         t.vt = vt;
         t.e_base = p.E;
@@ -1307,7 +1723,7 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         p.E += x.npts;
         p.K += t.bg.nkeys + 1;  // the job's keys and its sentinel
         p.max_npts = std::max<int64_t>(p.max_npts, x.npts);
-        const int64_t ntiles = (int64_t)t.g.ntx * t.g.ntx;
+        const int64_t ntiles = (int64_t)t.g.ntx * t.g.nty;
         t.off_split = off;
         off += align256((size_t)ntiles);
         t.off_tcnt = off;
@@ -1322,8 +1738,7 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
             p.nconv++;
             p.max_gcf = std::max<int64_t>(p.max_gcf, (int64_t)kSubKernels * x.S * x.S);
         }
-        t.taps = o.strategy == MIGRID_TILED && x.S <= tap_smax &&
-                 (tc.T + x.S - 2) / 16 + 1 <= kTapRows;
+        t.taps = taps;
         (t.taps ? item_cap_t : item_cap) += ntiles * kSubKernels;
         if (t.taps) {
             p.taps = true;
@@ -1377,9 +1792,10 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.target_t = o.vis_per_wave > 0 ? o.vis_per_wave : 2048;
     // gather / window: split tiles are summed by an ordered reduction, not
     // atomics.  Tile tx lists bins tx*T/H .. (tx*T + T + S - 2)/H (T is a
-    // multiple of H), so per axis a visibility's bin is listed by at most
-    // floor((T + S - 2) / T) + 1 tiles, and by m = that squared in all: the
-    // per-tile counts sum to at most m E.  A split tile holds c > target of
+    // multiple of H), so along x a visibility's bin is listed by at most
+    // floor((T + S - 2) / T) + 1 tiles, along y by floor((Ty + S - 2) / Ty)
+    // + 1, and by m = their product in all: the per-tile counts sum to at
+    // most m E.  A split tile holds c > target of
     // them and makes at most ceil(c / target) < 2c / target items, so
     // 2 m E / target + 1 partial tiles and m E / target + 1 split tiles bound
     // what the work-item pass lists.  When those slots would pass 4 GiB the
@@ -1388,11 +1804,14 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     if (p.det) {
         int64_t m = 1;
         for (const TiledJob& t : p.jobs) {
-            const int64_t per_axis = (t.g.T + t.g.S - 2) / t.g.T + 1;
-            m = std::max<int64_t>(m, per_axis * per_axis);
+            const int64_t mx = (t.g.T + t.g.S - 2) / t.g.T + 1;
+            const int64_t my = (t.g.Ty + t.g.S - 2) / t.g.Ty + 1;
+            m = std::max<int64_t>(m, mx * my);
         }
         const size_t tsz = (p.key.stage == kStageF64 || p.key.acc64) ? 8 : 4;  // accumulator
-        const int64_t slot = (int64_t)p.key.T * p.key.T * 2 * (int64_t)tsz;
+        int64_t tpx = 0;  // pixels of the largest tile
+        for (const TiledJob& t : p.jobs) tpx = std::max<int64_t>(tpx, (int64_t)t.g.T * t.g.Ty);
+        const int64_t slot = tpx * 2 * (int64_t)tsz;
         const int64_t max_slots = ((int64_t)4 << 30) / slot;
         if (2 * m * p.E / p.target + 1 > max_slots)
             p.target = (2 * m * p.E + max_slots - 2) / (max_slots - 1);
@@ -1410,7 +1829,9 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.stage_min = o.stage_min;
     p.wg_per_cu = o.wg_per_cu;
     const size_t gsz = p.key.stage == kStageBf16 ? 4 : p.key.stage == kStageF32 ? 8 : 16;
-    p.lds = p.owner ? (size_t)stage_rows(smax_owner) * stage_pitch(smax_owner) * gsz : 0;
+    p.lds = !p.owner ? 0
+            : p.key.T == 64 ? (size_t)col_rows(smax_owner) * col_pitch(smax_owner) * gsz
+                            : (size_t)stage_rows(smax_owner) * stage_pitch(smax_owner) * gsz;
     p.lds_taps = p.taps ? (size_t)2 * p.key.T * tap_pitch(p.key.T) * sizeof(double) : 0;
     return MIGRID_OK;
 }
@@ -1515,7 +1936,7 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
         pj[j].e_base = t.e_base;
         pj[j].nkeys = t.bg.nkeys;
         pj[j].S = t.a.S;
-        pj[j].pitch = stage_pitch(t.a.S);
+        pj[j].pitch = t.g.T == kColTx ? col_pitch(t.a.S) : stage_pitch(t.a.S);
         pj[j].esize = p.key.stage == kStageF64 ? 16 : p.key.stage == kStageF32 ? 8 : 4;
         if (t.convert) {
             ConvJob c;
@@ -1570,8 +1991,8 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
     }
     int64_t max_tiles = 0, max_ntx = 0;
     for (const TiledJob& t : p.jobs) {
-        max_tiles = std::max<int64_t>(max_tiles, (int64_t)t.g.ntx * t.g.ntx);
-        max_ntx = std::max<int64_t>(max_ntx, t.g.ntx);
+        max_tiles = std::max<int64_t>(max_tiles, (int64_t)t.g.ntx * t.g.nty);
+        max_ntx = std::max<int64_t>(max_ntx, std::max(t.g.ntx, t.g.nty));
     }
     const JobDesc* ddesc = (const JobDesc*)(arena + p.off_desc);
     const int cb = (int)std::min<int64_t>(max_ntx * kSubKernels, n < 8 ? 8192 : 65536 / n);

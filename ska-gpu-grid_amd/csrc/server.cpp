@@ -143,7 +143,7 @@ size_t input_bytes(const migrid_server_cfg& c, int64_t npts, int64_t S)
 // offsets, work items, converted GCF table.  An upper estimate for admission.
 size_t work_bytes(int64_t npts, int64_t N, int64_t S)
 {
-    const int64_t nb = (N + S - 2) / 16 + 1;
+    const int64_t nb = (N + S - 2) / 8 + 1;  // bins of 8 (64-pixel tiles)
     const int64_t tiles = (N + 15) / 16;
     return (size_t)npts * 48 + (size_t)kSubKernels * nb * nb * 8 + (size_t)tiles * tiles * 64 +
            (size_t)kSubKernels * S * S * 16 + (1 << 20);

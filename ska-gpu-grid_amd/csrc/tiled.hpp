@@ -52,10 +52,12 @@ struct ReduceItem {
 
 struct TileGeom {
     int N, S;
-    int T;    // tile edge (pixels)
+    int T;    // tile width (pixels)
     int H;    // bin edge (pixels)
     int nb;   // bins per axis
-    int ntx;  // tiles per axis
+    int ntx;  // tiles along x
+    int Ty;   // tile height: T, or 64 for the column kernel's 128 x 64 tiles
+    int nty;  // tiles along y
 };
 
 // Everything the owner kernel needs about one job (device-resident table,
@@ -105,10 +107,11 @@ struct KernelKey {
 };
 
 struct TileCfg {
-    int T;
+    int T;   // tile width
+    int Ty;  // tile height
     int H;
 };
-TileCfg pick_tile(int S, int tile_opt);
+TileCfg pick_tile(int S, int tile_opt, bool taps);
 
 struct TiledJob {
     GridArgs a;
