@@ -55,26 +55,12 @@ __global__ __launch_bounds__(256) void k_keys(const BinJob* __restrict__ jobs,
 {
     const BinJob jb = jobs[blockIdx.y];
     const TI* __restrict__ uv = (const TI*)jb.uv;
-    const int N = jb.N, S = jb.S, H = jb.H, nb = jb.nb;
+    const int nb = jb.nb;
     const uint32_t nkeys = (uint32_t)kSubKernels * nb * nb;
     const uint32_t kbase = (uint32_t)jb.k_base;
-    const int qmax = N + S - 2;
     for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < jb.npts;
-         n += (int64_t)gridDim.x * blockDim.x) {
-        const TI u = uv[2 * n], v = uv[2 * n + 1];
-        const TI fu = floor_t(u), fv = floor_t(v);
-        const int sx = (int)floor_t((TI)kOversample * (u - fu));
-        const int sy = (int)floor_t((TI)kOversample * (v - fv));
-        // q computed in the input type's range before narrowing: a coordinate
-        // far outside the grid must not wrap into it
-        const TI qxf = fu + (TI)(S / 2), qyf = fv + (TI)(S / 2);
-        uint32_t key = nkeys;
-        if (qxf >= 0 && qyf >= 0 && qxf <= (TI)qmax && qyf <= (TI)qmax) {
-            const int qx = (int)qxf, qy = (int)qyf;
-            key = ((uint32_t)(kOversample * sy + sx) * nb + qy / H) * nb + qx / H;
-        }
-        keys[jb.e_base + n] = kbase + key;
-    }
+         n += (int64_t)gridDim.x * blockDim.x)
+        keys[jb.e_base + n] = kbase + bin_key<TI>(uv[2 * n], uv[2 * n + 1], jb.N, jb.S, jb.H, nb, nkeys);
 }
 
 // First position of every present key: marks[key] = p where the sorted key
@@ -289,9 +275,18 @@ __global__ __launch_bounds__(256) void k_smin_reduce(const int32_t* __restrict__
     __shared__ int ws[4];
     const int64_t base = (int64_t)blockIdx.x * kScanTile;
     int m = INT_MAX;
-    for (int j = 0; j < kScanItems; j++) {
-        const int64_t i = base + j * 256 + threadIdx.x;
-        if (i < n) m = min(m, x[i]);
+    // the block's tile in 16-byte loads (kScanTile elements: 2 per thread),
+    // element by element at the array's end or when x is not 16-byte aligned
+    const bool vec = ((uintptr_t)x & 15) == 0;
+    for (int j = 0; j < kScanItems / 4; j++) {
+        const int64_t i = base + 4 * (j * 256 + threadIdx.x);
+        if (vec && i + 4 <= n) {
+            const int4 a = *(const int4*)(x + i);
+            m = min(m, min(min(a.x, a.y), min(a.z, a.w)));
+        } else {
+            for (int e = 0; e < 4; e++)
+                if (i + e < n) m = min(m, x[i + e]);
+        }
     }
     for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_down(m, o, 64));
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = m;
@@ -334,15 +329,23 @@ __global__ __launch_bounds__(1024) void k_smin_partials(int32_t* __restrict__ p,
 __global__ __launch_bounds__(256) void k_smin_final(int32_t* __restrict__ x, int64_t n,
                                                     const int32_t* __restrict__ partial)
 {
+    static_assert(kScanItems == 8, "two 16-byte loads per thread");
     __shared__ int ws[4];
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    // a thread's 8 consecutive elements by two 16-byte loads (base is a
+    // multiple of 8), element by element at the array's end or when x is not
+    // 16-byte aligned
+    const bool vec = ((uintptr_t)x & 15) == 0;
     int v[kScanItems];
-    int m = INT_MAX;
-    for (int j = kScanItems - 1; j >= 0; j--) {
-        const int64_t i = base + j;
-        v[j] = i < n ? x[i] : INT_MAX;
-        m = min(m, v[j]);
+    if (vec && base + kScanItems <= n) {
+        const int4 a = *(const int4*)(x + base), b = *(const int4*)(x + base + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+        for (int j = 0; j < kScanItems; j++) v[j] = base + j < n ? x[base + j] : INT_MAX;
     }
+    int m = INT_MAX;
+    for (int j = 0; j < kScanItems; j++) m = min(m, v[j]);
     // suffix min over the threads after this one, then the blocks after this one
     const int inc = wave_suffix_min(m);
     const int lane = threadIdx.x & 63;
@@ -353,9 +356,15 @@ __global__ __launch_bounds__(256) void k_smin_final(int32_t* __restrict__ x, int
     for (int w = (int)(threadIdx.x >> 6) + 1; w < 4; w++) run = min(run, ws[w]);
     run = min(run, partial[blockIdx.x]);
     for (int j = kScanItems - 1; j >= 0; j--) {
-        const int64_t i = base + j;
         run = min(run, v[j]);
-        if (i < n) x[i] = run;
+        v[j] = run;
+    }
+    if (vec && base + kScanItems <= n) {
+        *(int4*)(x + base) = make_int4(v[0], v[1], v[2], v[3]);
+        *(int4*)(x + base + 4) = make_int4(v[4], v[5], v[6], v[7]);
+    } else {
+        for (int j = 0; j < kScanItems; j++)
+            if (base + j < n) x[base + j] = v[j];
     }
 }
 

@@ -115,18 +115,22 @@ constexpr int kValCompatS = 1;  // value x COMPUTE_GCF tap value, scatter form
 constexpr int kValCompatI = 2;  // the same with the fixed-point dx of gather/window
 constexpr int kValDebug = 3;    // (1, n): the DEBUG1 coverage map
 
-// Records of every job in one launch (blockIdx.y = job): the job's binned
-// visibilities are the sorted positions [offs[0], offs[nkeys]).
+// Keys and records of every job's visibilities in input order (blockIdx.y =
+// job): the bin key (bin_key, the bin sort's key) into the sort's key buffer
+// and the record -- q, its offset in the staged layout and the value, with
+// the value transform of VT -- into rec0.  Both are read and written in
+// order; the sorted records are then one gather (k_gather), one record read
+// per visibility instead of its coordinates and value from two arrays.
 template <typename TI, typename R, int VT>
-__global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
-                                              const int32_t* __restrict__ perm, R* __restrict__ rec)
+__global__ __launch_bounds__(256) void k_keyrec(const PackJob* __restrict__ jobs,
+                                                uint32_t* __restrict__ keys, R* __restrict__ rec0)
 {
     const PackJob jb = jobs[blockIdx.y];
     const TI* __restrict__ uv = (const TI*)jb.uv;
     const TI* __restrict__ vals = (const TI*)jb.vals;
     const TI* __restrict__ gcf = (const TI*)jb.gcf;
     const int S = jb.S, pitch = jb.pitch, esize = jb.esize;
-    const int64_t p_lo = jb.offs[0], p_hi = jb.offs[jb.nkeys];
+    const uint32_t nkeys = (uint32_t)jb.nkeys, kbase = (uint32_t)jb.k_base;
     float p1 = 0.f, p2 = 0.f;
     if (VT == kValCompatS || VT == kValCompatI) {
         const int64_t e = (int64_t)S * (S - 1) / 2 - 1;  // grid_gpu.cu:104-107
@@ -134,10 +138,12 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
         p1 = (float)(2 * 3.1415926 * w);
         p2 = (float)(p1 * T);
     }
-    for (int64_t j = p_lo + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < p_hi;
-         j += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t n = perm[j] - jb.e_base;
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < jb.npts;
+         n += (int64_t)gridDim.x * blockDim.x) {
         const TI u = uv[2 * n], v = uv[2 * n + 1];
+        keys[jb.e_base + n] = kbase + bin_key<TI>(u, v, jb.N, S, jb.H, jb.nb, nkeys);
+        // a visibility outside the grid keeps the sentinel key and its
+        // record is never gathered
         const TI fu = floor_t(u), fv = floor_t(v);
         const int mx = (int)fu, my = (int)fv;
         const uint32_t qx = (uint32_t)(mx + S / 2), qy = (uint32_t)(my + S / 2);
@@ -173,8 +179,23 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
         r.off = (qy * (uint32_t)pitch + qx) * (uint32_t)esize;
         r.vr = (decltype(r.vr))vr;
         r.vi = (decltype(r.vi))vi;
-        rec[j] = r;
+        rec0[jb.e_base + n] = r;
     }
+}
+
+// Sorted records of every job (blockIdx.y = job): the job's binned
+// visibilities are the sorted positions [offs[0], offs[nkeys]), and position
+// j takes the record of element perm[j].
+template <typename R>
+__global__ __launch_bounds__(256) void k_gather(const PackJob* __restrict__ jobs,
+                                                const int32_t* __restrict__ perm,
+                                                const R* __restrict__ rec0, R* __restrict__ rec)
+{
+    const PackJob jb = jobs[blockIdx.y];
+    const int64_t p_lo = jb.offs[0], p_hi = jb.offs[jb.nkeys];
+    for (int64_t j = p_lo + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < p_hi;
+         j += (int64_t)gridDim.x * blockDim.x)
+        rec[j] = rec0[perm[j]];
 }
 
 // ---- per-tile counts ----------------------------------------------------------------
@@ -1579,21 +1600,31 @@ int run_taps_layout(const KernelKey& k, const OwnerLaunch& L, hipStream_t s)
 }
 
 template <typename TI, typename R>
-void launch_pack(const PackJob* table, int n, int64_t max_npts, int vt, const int32_t* perm, R* rec,
-                 hipStream_t s)
+void launch_keyrec(const PackJob* table, int n, int64_t max_npts, int vt, uint32_t* keys, R* rec0,
+                   hipStream_t s)
 {
     const int64_t per = (max_npts + 255) / 256;
     const int64_t cap = n < 8192 ? 8192 / n : 1;
     const int bx = (int)(per < 1 ? 1 : per < cap ? per : cap);
-#define MG_PACK(VT)                                                                            \
-    hipLaunchKernelGGL((k_pack<TI, R, VT>), dim3(bx, n), dim3(256), 0, s, table, perm, rec)
+#define MG_KEYREC(VT)                                                                          \
+    hipLaunchKernelGGL((k_keyrec<TI, R, VT>), dim3(bx, n), dim3(256), 0, s, table, keys, rec0)
     switch (vt) {
-    case kValCompatS: MG_PACK(kValCompatS); break;
-    case kValCompatI: MG_PACK(kValCompatI); break;
-    case kValDebug: MG_PACK(kValDebug); break;
-    default: MG_PACK(kValPlain);
+    case kValCompatS: MG_KEYREC(kValCompatS); break;
+    case kValCompatI: MG_KEYREC(kValCompatI); break;
+    case kValDebug: MG_KEYREC(kValDebug); break;
+    default: MG_KEYREC(kValPlain);
     }
-#undef MG_PACK
+#undef MG_KEYREC
+}
+
+template <typename R>
+void launch_gather(const PackJob* table, int n, int64_t max_npts, const int32_t* perm,
+                   const R* rec0, R* rec, hipStream_t s)
+{
+    const int64_t per = (max_npts + 255) / 256;
+    const int64_t cap = n < 8192 ? 8192 / n : 1;
+    const int bx = (int)(per < 1 ? 1 : per < cap ? per : cap);
+    hipLaunchKernelGGL((k_gather<R>), dim3(bx, n), dim3(256), 0, s, table, perm, rec0, rec);
 }
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -1764,8 +1795,8 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     off += align256((size_t)p.K * 4);
     p.off_rec = off;
     off += align256((size_t)(p.E > 0 ? p.E : 1) * (rec64 ? sizeof(RecF64) : sizeof(RecF32)));
-    p.off_bjobs = off;
-    off += align256(sizeof(BinJob) * n);
+    p.off_rec0 = off;  // the records in input order, before the gather
+    off += align256((size_t)(p.E > 0 ? p.E : 1) * (rec64 ? sizeof(RecF64) : sizeof(RecF32)));
     p.off_pjobs = off;
     off += align256(sizeof(PackJob) * n);
     p.off_cjobs = off;
@@ -1916,26 +1947,22 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
     int32_t* offs = (int32_t*)(arena + p.off_offs);
     MG_TRY(fill_u32(qs, 0u, 2 * sizeof(QueueState) / 4, s));  // both queues
     // job tables
-    std::vector<BinJob> bj(n);
     std::vector<PackJob> pj(n);
     std::vector<ConvJob> cj;
     for (int j = 0; j < n; j++) {
         const TiledJob& t = p.jobs[j];
-        bj[j].uv = t.a.uv;
-        bj[j].npts = t.a.npts;
-        bj[j].e_base = t.e_base;
-        bj[j].k_base = t.k_base;
-        bj[j].N = t.bg.N;
-        bj[j].S = t.bg.S;
-        bj[j].H = t.bg.H;
-        bj[j].nb = t.bg.nb;
         pj[j].uv = t.a.uv;
         pj[j].vals = t.a.vals;
         pj[j].gcf = t.a.gcf;
         pj[j].offs = offs + t.k_base;
         pj[j].e_base = t.e_base;
+        pj[j].npts = t.a.npts;
+        pj[j].k_base = t.k_base;
         pj[j].nkeys = t.bg.nkeys;
+        pj[j].N = t.bg.N;
         pj[j].S = t.a.S;
+        pj[j].H = t.bg.H;
+        pj[j].nb = t.bg.nb;
         pj[j].pitch = t.g.T == kColTx ? col_pitch(t.a.S) : stage_pitch(t.a.S);
         pj[j].esize = p.key.stage == kStageF64 ? 16 : p.key.stage == kStageF32 ? 8 : 4;
         if (t.convert) {
@@ -1946,9 +1973,7 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
             cj.push_back(c);
         }
     }
-    BinJob* dbj = (BinJob*)(arena + p.off_bjobs);
     PackJob* dpj = (PackJob*)(arena + p.off_pjobs);
-    MG_TRY(put_table(dbj, bj.data(), n, s));
     MG_TRY(put_table(dpj, pj.data(), n, s));
     if (!cj.empty()) {
         ConvJob* dcj = (ConvJob*)(arena + p.off_cjobs);
@@ -1956,17 +1981,29 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
         MG_TRY(launch_convert(dcj, (int)cj.size(), p.max_gcf, p.key.stage,
                               p.key.in_type == MIGRID_F64, s));
     }
-    // the jobs sorted here come first; the others' kept sorts are moved to
-    // their places in this launch
+    // every job's keys (into the sort's key buffer) and records, in input order
+    void* rec = arena + p.off_rec;
+    void* rec0 = arena + p.off_rec0;
+    uint32_t* keys = sort_key_buffer(bin_scratch);
+    if (p.E > 0) {
+        if (p.key.in_type == MIGRID_F64) {
+            if (p.key.stage == kStageF64)
+                launch_keyrec<double, RecF64>(dpj, n, p.max_npts, p.vt, keys, (RecF64*)rec0, s);
+            else
+                launch_keyrec<double, RecF32>(dpj, n, p.max_npts, p.vt, keys, (RecF32*)rec0, s);
+        } else {
+            launch_keyrec<float, RecF32>(dpj, n, p.max_npts, p.vt, keys, (RecF32*)rec0, s);
+        }
+        MG_HIP(hipGetLastError());
+    }
+    // the jobs sorted here come first (their keys are the buffer's first
+    // elements); the others' kept sorts are moved to their places in this launch
     if (n_sorted == n) {
-        MG_TRY(bin_sort(dbj, n, p.max_npts, p.E, p.K, p.key.in_type, bin_scratch, perm, offs, s));
+        MG_TRY(sort_keys(p.E, p.K, bin_scratch, perm, offs, s));
     } else {
         if (n_sorted > 0) {
             const TiledJob& first = p.jobs[n_sorted];
-            int64_t mx = 0;
-            for (int j = 0; j < n_sorted; j++) mx = std::max<int64_t>(mx, p.jobs[j].a.npts);
-            MG_TRY(bin_sort(dbj, n_sorted, mx, first.e_base, first.k_base, p.key.in_type,
-                            bin_scratch, perm, offs, s));
+            MG_TRY(sort_keys(first.e_base, first.k_base, bin_scratch, perm, offs, s));
         }
         for (int j = n_sorted; j < n; j++) {
             const TiledJob& t = p.jobs[j];
@@ -1978,15 +2015,10 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
         }
     }
     if (p.E > 0) {
-        void* rec = arena + p.off_rec;
-        if (p.key.in_type == MIGRID_F64) {
-            if (p.key.stage == kStageF64)
-                launch_pack<double, RecF64>(dpj, n, p.max_npts, p.vt, perm, (RecF64*)rec, s);
-            else
-                launch_pack<double, RecF32>(dpj, n, p.max_npts, p.vt, perm, (RecF32*)rec, s);
-        } else {
-            launch_pack<float, RecF32>(dpj, n, p.max_npts, p.vt, perm, (RecF32*)rec, s);
-        }
+        if (p.key.stage == kStageF64)
+            launch_gather<RecF64>(dpj, n, p.max_npts, perm, (const RecF64*)rec0, (RecF64*)rec, s);
+        else
+            launch_gather<RecF32>(dpj, n, p.max_npts, perm, (const RecF32*)rec0, (RecF32*)rec, s);
         MG_HIP(hipGetLastError());
     }
     int64_t max_tiles = 0, max_ntx = 0;

@@ -127,16 +127,21 @@ struct TiledJob {
     bool taps = false;  // small support: gridded by k_taps
 };
 
-// Record packing of one job (device table, blockIdx.y of k_pack): the job's
-// binned visibilities are the sorted positions [offs[0], offs[nkeys]).
+// Keys and records of one job (device table, blockIdx.y of k_keyrec and
+// k_gather): its elements are [e_base, e_base + npts) of the concatenated
+// arrays, its keys [k_base, k_base + nkeys]; its binned visibilities are the
+// sorted positions [offs[0], offs[nkeys]).
 struct PackJob {
     const void* uv;
     const void* vals;
     const void* gcf;
     const int32_t* offs;
     int64_t e_base;
+    int64_t npts;
+    int64_t k_base;
     int64_t nkeys;
-    int S, pitch;
+    int N, S, H, nb;
+    int pitch;
     int esize;  // bytes of one staged element: record offsets are in bytes
 };
 
@@ -162,7 +167,7 @@ struct TiledPlan {
     bool fresh = false;    // fresh grids (cleared by the plan's own passes)
     bool shared = false;   // jobs share grids (atomic flushes; the caller clears)
     size_t off_perm = 0, off_offs = 0, off_rec = 0;
-    size_t off_bjobs = 0, off_pjobs = 0, off_cjobs = 0;
+    size_t off_rec0 = 0, off_pjobs = 0, off_cjobs = 0;
     int nconv = 0;             // jobs whose GCF is converted
     int64_t max_gcf = 0;       // largest GCF table (complex elements)
     size_t off_desc = 0, off_items = 0, off_qs = 0, arena_bytes = 0;
