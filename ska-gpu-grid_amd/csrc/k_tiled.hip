@@ -226,6 +226,20 @@ __global__ __launch_bounds__(128) void k_tile_counts(const JobDesc* __restrict__
     }
 }
 
+// A pixel's (re, im) pair of an interleaved grid, stored or added as one
+// 16- (double) or 8-byte (float) access: a wave's 8-pixel row segment then
+// writes whole 128-byte lines, not half of each line per instruction.
+template <typename TO>
+struct Pair2;
+template <>
+struct Pair2<double> {
+    using T = double2;
+};
+template <>
+struct Pair2<float> {
+    using T = float2;
+};
+
 // Zeroes the pixels of one tile (the whole wave; lanes along a row).
 template <typename TO>
 __device__ void zero_tile(TO* grid, int N, int X0, int Y0, int T, int Ty, bool planar)
@@ -240,8 +254,9 @@ __device__ void zero_tile(TO* grid, int N, int X0, int Y0, int T, int Ty, bool p
                 grid[p] = (TO)0;
                 grid[NN + p] = (TO)0;
             } else {
-                grid[2 * p] = (TO)0;
-                grid[2 * p + 1] = (TO)0;
+                typename Pair2<TO>::T z;
+                z.x = z.y = (TO)0;
+                *(typename Pair2<TO>::T*)(grid + 2 * p) = z;
             }
         }
 }
@@ -362,6 +377,22 @@ template <typename TO, typename TA>
 __device__ __forceinline__ void flush_px(TO* grid, TO* lo, int64_t idx_re, int64_t idx_im, TA re,
                                          TA im, bool atomic, bool fresh)
 {
+    if (!atomic && lo == nullptr && idx_im == idx_re + 1) {
+        // interleaved, the tile this item's alone
+        using V2 = typename Pair2<TO>::T;
+        V2* q = (V2*)(grid + idx_re);
+        V2 v;
+        if (fresh) {
+            v.x = (TO)re;
+            v.y = (TO)im;
+        } else {
+            v = *q;
+            v.x += (TO)re;
+            v.y += (TO)im;
+        }
+        *q = v;
+        return;
+    }
     if (!atomic && fresh) {
         grid[idx_re] = (TO)re;
         grid[idx_im] = (TO)im;
@@ -1818,7 +1849,10 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         p.shared = std::adjacent_find(grids.begin(), grids.end()) != grids.end();
     }
     p.fresh = !o.accumulate && !p.shared;
-    p.target = o.vis_per_wave > 0 ? o.vis_per_wave : 4096;
+    // 8192: at 8192^2 / S=64 / 10M half the split tiles of 4096 (whose
+    // flushes are atomics) and no slower; larger items leave dense tiles to
+    // too few workgroups
+    p.target = o.vis_per_wave > 0 ? o.vis_per_wave : 8192;
     // the tap kernel's items: smaller by default, so dense tiles spread wider
     p.target_t = o.vis_per_wave > 0 ? o.vis_per_wave : 2048;
     // gather / window: split tiles are summed by an ordered reduction, not
