@@ -999,7 +999,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner_col(
                                 ? jobs[job].tcount[(int64_t)tile * kSubKernels + lane]
                                 : 0;
             nz_mask = __ballot(cnt > 0);
-            st_mask = (GM != 0 || SC != 0) ? nz_mask : __ballot(cnt >= stage_min && cnt > 0);
+            // (supports 32 and 64 stage every non-empty sub-kernel)
+            st_mask = (GM != 0 || SC != 0 || S == 32 || S == 64)
+                          ? nz_mask
+                          : __ballot(cnt >= stage_min && cnt > 0);
         }
         // sub-kernel rows this wave stages: w, w + NW, ... (one column per
         // lane); the next staged sub-kernel is fetched into registers while
@@ -1081,58 +1084,68 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner_col(
                 }
                 __syncthreads();
             }
+            // the support a constant SX: each bin row's groups and the group
+            // offsets are constants
+            auto rows_const = [&](auto sx) {
+                constexpr int SX = decltype(sx)::value;
+                constexpr int NR = col_nrows(SX), DC = col_depth(SX), PC = col_pitch(SX);
+                constexpr uint32_t RS = sizeof(R);
+                ConstPtr<char> rbase = (ConstPtr<char>)rec;
+                // the bounds as byte offsets into the records, and a mask
+                // of the non-empty rows (bit 2t: row t), so an empty row
+                // costs one bit test
+                const uint32_t bb = (uint32_t)bnd * RS;
+                const uint32_t bb_next =
+                    (uint32_t)__builtin_amdgcn_mov_dpp((int)bb, 0xB1, 0xF, 0xF, false);
+                const uint64_t rows_on = __ballot(!(lane & 1) && bb < bb_next);
+                static_for<NR>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr int A = t - DC > 0 ? t - DC : 0;
+                    constexpr int B = t < NG - 1 ? t : NG - 1;
+                    if (!((rows_on >> (2 * t)) & 1)) return;
+                    uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)bb, 2 * t);
+                    const uint32_t oe = (uint32_t)__builtin_amdgcn_readlane((int)bb, 2 * t + 1);
+                    // the taps of one visibility
+                    auto one = [&](uint32_t off, V vr, V vi) {
+                        const G* b = (const G*)(lb - off);
+                        P gr[NG], gi[NG];
+#pragma unroll
+                        for (int i = A; i <= B; i++) Stage<G>::get(b[8 * i * PC], gr[i], gi[i]);
+#pragma unroll
+                        for (int i = A; i <= B; i++)
+                            mac1<P, TA>(acc[i], (P)vr, (P)vi, gr[i], gi[i]);
+                    };
+                    // pairs of visibilities from one scalar record load,
+                    // then the row's odd last one alone
+                    const uint32_t oe1 = oe - RS;
+                    for (; o < oe1; o += 2 * RS) {
+                        uint32_t off[2];
+                        V vr[2], vi[2];
+                        pair_at<R>(rbase, o, off, vr, vi);
+                        one(off[0], vr[0], vi[0]);
+                        // rows of many groups: the second visibility's
+                        // taps are read after the first one's products,
+                        // so one visibility's taps are live at a time
+                        if constexpr (B - A + 1 > 4) __builtin_amdgcn_sched_barrier(0);
+                        one(off[1], vr[1], vi[1]);
+                    }
+                    if (o == oe1) {
+                        uint32_t off;
+                        V vr, vi;
+                        rec_at<R>(rbase, o, off, vr, vi);
+                        one(off, vr, vi);
+                    }
+                });
+            };
             if (staged) {
                 if constexpr (SC != 0) {
-                    // the support a constant: each bin row's groups and the
-                    // group offsets are constants
-                    constexpr int NR = col_nrows(SC), DC = col_depth(SC), PC = col_pitch(SC);
-                    constexpr uint32_t RS = sizeof(R);
-                    ConstPtr<char> rbase = (ConstPtr<char>)rec;
-                    // the bounds as byte offsets into the records, and a mask
-                    // of the non-empty rows (bit 2t: row t), so an empty row
-                    // costs one bit test
-                    const uint32_t bb = (uint32_t)bnd * RS;
-                    const uint32_t bb_next =
-                        (uint32_t)__builtin_amdgcn_mov_dpp((int)bb, 0xB1, 0xF, 0xF, false);
-                    const uint64_t rows_on = __ballot(!(lane & 1) && bb < bb_next);
-                    static_for<NR>([&](auto tc) {
-                        constexpr int t = decltype(tc)::value;
-                        constexpr int A = t - DC > 0 ? t - DC : 0;
-                        constexpr int B = t < NG - 1 ? t : NG - 1;
-                        if (!((rows_on >> (2 * t)) & 1)) return;
-                        uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)bb, 2 * t);
-                        const uint32_t oe = (uint32_t)__builtin_amdgcn_readlane((int)bb, 2 * t + 1);
-                        // the taps of one visibility
-                        auto one = [&](uint32_t off, V vr, V vi) {
-                            const G* b = (const G*)(lb - off);
-                            P gr[NG], gi[NG];
-#pragma unroll
-                            for (int i = A; i <= B; i++) Stage<G>::get(b[8 * i * PC], gr[i], gi[i]);
-#pragma unroll
-                            for (int i = A; i <= B; i++)
-                                mac1<P, TA>(acc[i], (P)vr, (P)vi, gr[i], gi[i]);
-                        };
-                        // pairs of visibilities from one scalar record load,
-                        // then the row's odd last one alone
-                        const uint32_t oe1 = oe - RS;
-                        for (; o < oe1; o += 2 * RS) {
-                            uint32_t off[2];
-                            V vr[2], vi[2];
-                            pair_at<R>(rbase, o, off, vr, vi);
-                            one(off[0], vr[0], vi[0]);
-                            // rows of many groups: the second visibility's
-                            // taps are read after the first one's products,
-                            // so one visibility's taps are live at a time
-                            if constexpr (B - A + 1 > 4) __builtin_amdgcn_sched_barrier(0);
-                            one(off[1], vr[1], vi[1]);
-                        }
-                        if (o == oe1) {
-                            uint32_t off;
-                            V vr, vi;
-                            rec_at<R>(rbase, o, off, vr, vi);
-                            one(off, vr, vi);
-                        }
-                    });
+                    rows_const(std::integral_constant<int, SC>{});
+                } else if (S == 64) {
+                    // jobs of mixed supports: the common ones take the
+                    // constant-support rows, decided once per item
+                    rows_const(std::integral_constant<int, 64>{});
+                } else if (S == 32) {
+                    rows_const(std::integral_constant<int, 32>{});
                 } else {
                     // any support: the groups of bin row t are tested per
                     // visibility (wave-uniform branches)
