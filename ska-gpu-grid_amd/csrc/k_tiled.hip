@@ -999,10 +999,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner_col(
                                 ? jobs[job].tcount[(int64_t)tile * kSubKernels + lane]
                                 : 0;
             nz_mask = __ballot(cnt > 0);
-            // (supports 32 and 64 stage every non-empty sub-kernel)
-            st_mask = (GM != 0 || SC != 0 || S == 32 || S == 64)
-                          ? nz_mask
-                          : __ballot(cnt >= stage_min && cnt > 0);
+            // (supports 32 and 64 stage every non-empty sub-kernel unless a
+            // threshold is given; stage_min 0 is the default)
+            const bool all = GM != 0 || SC != 0 || ((S == 32 || S == 64) && stage_min == 0);
+            const int smin = stage_min > 0 ? stage_min : kStageMinDefault;
+            st_mask = all ? nz_mask : __ballot(cnt >= smin && cnt > 0);
         }
         // sub-kernel rows this wave stages: w, w + NW, ... (one column per
         // lane); the next staged sub-kernel is fetched into registers while
@@ -1559,7 +1560,7 @@ int run_owner_col(const OwnerLaunch& L, hipStream_t s)
     }
     if (L.wg_per_cu > 0) per_cu = L.wg_per_cu;
     hipLaunchKernelGGL(kern, dim3(ncu * per_cu), dim3(1024), L.lds, s, L.jobs, L.items, L.qs,
-                       L.smax, L.stage_min > 0 ? L.stage_min : kStageMinDefault);
+                       L.smax, L.stage_min);
     MG_HIP(hipGetLastError());
     return MIGRID_OK;
 }
@@ -1806,7 +1807,16 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         // a table in the staged type unless the caller's table already is one
         t.convert = k.gm == 0 && !((k.stage == kStageF64 && x.in_type == MIGRID_F64) ||
                                    (k.stage == kStageF32 && x.in_type == MIGRID_F32));
-        if (t.convert) {
+        // jobs of one table (same source and support) share its conversion
+        for (int i = 0; t.convert && i < j; i++) {
+            const TiledJob& u = p.jobs[i];
+            if (u.convert && !u.conv_dup && u.a.gcf == x.gcf && u.a.S == x.S) {
+                t.off_gtab = u.off_gtab;
+                t.conv_dup = true;
+                break;
+            }
+        }
+        if (t.convert && !t.conv_dup) {
             const size_t gsz = k.stage == kStageBf16 ? 4 : k.stage == kStageF32 ? 8 : 16;
             t.off_gtab = off;
             off += align256((size_t)kSubKernels * x.S * x.S * gsz);
@@ -2012,7 +2022,7 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
         pj[j].nb = t.bg.nb;
         pj[j].pitch = t.g.T == kColTx ? col_pitch(t.a.S) : stage_pitch(t.a.S);
         pj[j].esize = p.key.stage == kStageF64 ? 16 : p.key.stage == kStageF32 ? 8 : 4;
-        if (t.convert) {
+        if (t.convert && !t.conv_dup) {
             ConvJob c;
             c.src = t.a.gcf;
             c.dst = arena + t.off_gtab;

@@ -123,6 +123,7 @@ struct TiledJob {
     int64_t k_base = 0;  // first key (its keys are [k_base, k_base + nkeys])
     size_t off_split = 0, off_tcnt = 0;  // in the arena
     bool convert = false;  // the GCF table is converted to the staged type
+    bool conv_dup = false;  // ... by an earlier job of the same table (shared copy)
     size_t off_gtab = 0;
     bool taps = false;  // small support: gridded by k_taps
 };
