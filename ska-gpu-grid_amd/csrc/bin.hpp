@@ -58,7 +58,5 @@ int group_by_index(const int32_t* idx, int64_t npts, int ngcf, int in_type, cons
 int64_t scan_scratch_elems(int64_t n);
 int exclusive_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* partial, bool write_total,
                    hipStream_t s);
-// x[i] = min(x[i..n)), in place.
-int suffix_min(int32_t* x, int64_t n, int32_t* partial, hipStream_t s);
 
 }  // namespace migrid

@@ -14,10 +14,12 @@
 //      popcount of the peers below it, __builtin_amdgcn_mbcnt) and an LDS
 //      prefix over the block's waves; the first pass takes the element index
 //      as its value;
-//   3. CSR offsets (the bookmarks) from the sorted keys: the first position
-//      of every present key is marked, and a suffix minimum fills the keys
-//      that are absent -- offsets[k] = number of elements with key < k,
-//      without a histogram of atomics.
+//   3. CSR offsets (the bookmarks) from the sorted keys, offsets[k] = number
+//      of elements with key < k, written once per key: the key space is cut
+//      into chunks of 2048 keys, each chunk's first sorted position is found
+//      from the element where it begins, and per chunk the first position of
+//      every present key is marked in LDS and a suffix minimum fills the
+//      keys that are absent -- no histogram of atomics.
 // The permutation is the stable order, so it is bit-identical to a serial
 // counting sort of the same keys.
 #include <hip/hip_runtime.h>
@@ -61,18 +63,6 @@ __global__ __launch_bounds__(256) void k_keys(const BinJob* __restrict__ jobs,
     for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < jb.npts;
          n += (int64_t)gridDim.x * blockDim.x)
         keys[jb.e_base + n] = kbase + bin_key<TI>(uv[2 * n], uv[2 * n + 1], jb.N, jb.S, jb.H, nb, nkeys);
-}
-
-// First position of every present key: marks[key] = p where the sorted key
-// changes (marks starts filled with the element count).
-__global__ __launch_bounds__(256) void k_marks(const uint32_t* __restrict__ sk, int64_t n,
-                                               int32_t* __restrict__ marks)
-{
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n;
-         p += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t k = sk[p];
-        if (p == 0 || sk[p - 1] != k) marks[k] = (int32_t)p;
-    }
 }
 
 // Per-block digit counts, stored digit-major: counts[d * nblk + blk].
@@ -257,8 +247,7 @@ __global__ __launch_bounds__(256) void k_scan_final(const int32_t* __restrict__ 
     }
 }
 
-// ---- suffix minimum of int32 (in place) ------------------------------------
-// x[i] = min(x[i], x[i+1], ..., x[n-1]).
+// Inclusive suffix minimum over a wave's lanes (lane l: min over lanes l..63).
 __device__ __forceinline__ int wave_suffix_min(int x)
 {
     const int lane = threadIdx.x & 63;
@@ -269,102 +258,87 @@ __device__ __forceinline__ int wave_suffix_min(int x)
     return x;
 }
 
-__global__ __launch_bounds__(256) void k_smin_reduce(const int32_t* __restrict__ x, int64_t n,
-                                                     int32_t* __restrict__ partial)
+// ---- CSR offsets: offsets[j] = lower bound of key j in the sorted keys ------
+// Key space in chunks of kScanTile keys (one workgroup each).  First, for
+// every chunk c, start[c] = the first sorted position whose key is >= c C
+// (start is filled with the element count beforehand): the element at p
+// writes the chunks whose first key lies in (sk[p - 1], sk[p]].
+__global__ __launch_bounds__(256) void k_chunk_starts(const uint32_t* __restrict__ sk, int64_t n,
+                                                      int32_t* __restrict__ start)
 {
-    __shared__ int ws[4];
-    const int64_t base = (int64_t)blockIdx.x * kScanTile;
-    int m = INT_MAX;
-    // the block's tile in 16-byte loads (kScanTile elements: 2 per thread),
-    // element by element at the array's end or when x is not 16-byte aligned
-    const bool vec = ((uintptr_t)x & 15) == 0;
-    for (int j = 0; j < kScanItems / 4; j++) {
-        const int64_t i = base + 4 * (j * 256 + threadIdx.x);
-        if (vec && i + 4 <= n) {
-            const int4 a = *(const int4*)(x + i);
-            m = min(m, min(min(a.x, a.y), min(a.z, a.w)));
-        } else {
-            for (int e = 0; e < 4; e++)
-                if (i + e < n) m = min(m, x[i + e]);
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_down(m, o, 64));
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) partial[blockIdx.x] = min(min(ws[0], ws[1]), min(ws[2], ws[3]));
-}
-
-// Single block: partial[b] = min over the blocks after b (exclusive).
-__global__ __launch_bounds__(1024) void k_smin_partials(int32_t* __restrict__ p, int64_t n)
-{
-    __shared__ int ws[16];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = INT_MAX;
-    __syncthreads();
-    for (int64_t b = n - 1; b >= 0; b -= 1024) {
-        // chunk [b - 1023, b], thread t takes element b - t (descending)
-        const int64_t i = b - threadIdx.x;
-        const int x = i >= 0 ? p[i] : INT_MAX;
-        // inclusive prefix min in thread order = suffix min in element order
-        const int lane = threadIdx.x & 63;
-        int inc = x;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(inc, o, 64);
-            if (lane >= o) inc = min(inc, y);
-        }
-        if (lane == 63) ws[threadIdx.x >> 6] = inc;
-        __syncthreads();
-        int wmin = INT_MAX;
-        for (int w = 0; w < (int)(threadIdx.x >> 6); w++) wmin = min(wmin, ws[w]);
-        const int c = carry;
-        const int prev = __shfl_up(inc, 1, 64);  // all lanes take part
-        const int excl_in_chunk = min(wmin, lane ? prev : INT_MAX);
-        if (i >= 0) p[i] = min(c, excl_in_chunk);
-        __syncthreads();
-        if (threadIdx.x == 1023) carry = min(c, min(wmin, inc));
-        __syncthreads();
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t hi = sk[p] / kScanTile;
+        const int64_t lo = p == 0 ? 0 : (int64_t)(sk[p - 1] / kScanTile) + 1;
+        for (int64_t c = lo; c <= hi; c++) start[c] = (int32_t)p;
     }
 }
 
-__global__ __launch_bounds__(256) void k_smin_final(int32_t* __restrict__ x, int64_t n,
-                                                    const int32_t* __restrict__ partial)
+// Then each workgroup writes its chunk's offsets once: the first position of
+// every key present in the chunk goes to LDS (positions [start[c], start[c +
+// 1]) hold exactly the chunk's keys), and a suffix minimum over the chunk,
+// seeded with start[c + 1], gives every key's lower bound.  A chunk holding
+// more than kLbScan elements (many visibilities on a few keys) takes a binary
+// search per key instead of the scan.
+constexpr int64_t kLbScan = 16 * kScanTile;
+
+__global__ __launch_bounds__(256) void k_lower_bounds(const uint32_t* __restrict__ sk,
+                                                      int64_t K, const int32_t* __restrict__ start,
+                                                      int32_t* __restrict__ offs)
 {
-    static_assert(kScanItems == 8, "two 16-byte loads per thread");
+    static_assert(kScanItems == 8, "two 16-byte stores per thread");
+    __shared__ int32_t s_first[kScanTile];
     __shared__ int ws[4];
-    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-    // a thread's 8 consecutive elements by two 16-byte loads (base is a
-    // multiple of 8), element by element at the array's end or when x is not
-    // 16-byte aligned
-    const bool vec = ((uintptr_t)x & 15) == 0;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int64_t c = blockIdx.x;
+    const int64_t j0 = c * kScanTile;
+    const int32_t i0 = start[c], i1 = start[c + 1];
+    if ((int64_t)i1 - i0 > kLbScan) {
+        for (int e = tid; e < kScanTile && j0 + e < K; e += 256) {
+            const uint32_t j = (uint32_t)(j0 + e);
+            int32_t lo = i0, hi = i1;
+            while (lo < hi) {
+                const int32_t mid = lo + (hi - lo) / 2;
+                if (sk[mid] < j)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            offs[j0 + e] = lo;
+        }
+        return;
+    }
+    for (int e = tid; e < kScanTile; e += 256) s_first[e] = i1;
+    __syncthreads();
+    for (int32_t p = i0 + tid; p < i1; p += 256) {
+        const uint32_t k = sk[p];
+        if (p == 0 || sk[p - 1] != k) s_first[k - (uint32_t)j0] = p;
+    }
+    __syncthreads();
+    // thread t: keys [8 t, 8 t + 8) of the chunk
     int v[kScanItems];
-    if (vec && base + kScanItems <= n) {
-        const int4 a = *(const int4*)(x + base), b = *(const int4*)(x + base + 4);
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-    } else {
-        for (int j = 0; j < kScanItems; j++) v[j] = base + j < n ? x[base + j] : INT_MAX;
-    }
     int m = INT_MAX;
-    for (int j = 0; j < kScanItems; j++) m = min(m, v[j]);
-    // suffix min over the threads after this one, then the blocks after this one
+    for (int e = 0; e < kScanItems; e++) {
+        v[e] = s_first[kScanItems * tid + e];
+        m = min(m, v[e]);
+    }
     const int inc = wave_suffix_min(m);
-    const int lane = threadIdx.x & 63;
-    if (lane == 0) ws[threadIdx.x >> 6] = inc;
+    if (lane == 0) ws[tid >> 6] = inc;
     __syncthreads();
     const int later_in_wave = __shfl_down(inc, 1, 64);
     int run = lane < 63 ? later_in_wave : INT_MAX;
-    for (int w = (int)(threadIdx.x >> 6) + 1; w < 4; w++) run = min(run, ws[w]);
-    run = min(run, partial[blockIdx.x]);
-    for (int j = kScanItems - 1; j >= 0; j--) {
-        run = min(run, v[j]);
-        v[j] = run;
+    for (int w = (tid >> 6) + 1; w < 4; w++) run = min(run, ws[w]);
+    for (int e = kScanItems - 1; e >= 0; e--) {
+        run = min(run, v[e]);
+        v[e] = run;
     }
-    if (vec && base + kScanItems <= n) {
-        *(int4*)(x + base) = make_int4(v[0], v[1], v[2], v[3]);
-        *(int4*)(x + base + 4) = make_int4(v[4], v[5], v[6], v[7]);
+    const int64_t base = j0 + kScanItems * tid;
+    if (((uintptr_t)offs & 15) == 0 && base + kScanItems <= K) {
+        *(int4*)(offs + base) = make_int4(v[0], v[1], v[2], v[3]);
+        *(int4*)(offs + base + 4) = make_int4(v[4], v[5], v[6], v[7]);
     } else {
-        for (int j = 0; j < kScanItems; j++)
-            if (base + j < n) x[base + j] = v[j];
+        for (int e = 0; e < kScanItems; e++)
+            if (base + e < K) offs[base + e] = v[e];
     }
 }
 
@@ -372,16 +346,6 @@ __global__ __launch_bounds__(256) void k_smin_final(int32_t* __restrict__ x, int
 
 int64_t scan_scratch_elems(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
-int suffix_min(int32_t* x, int64_t n, int32_t* partial, hipStream_t s)
-{
-    if (n <= 0) return MIGRID_OK;
-    const int64_t nblk = (n + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(k_smin_reduce, dim3(nblk), dim3(256), 0, s, x, n, partial);
-    hipLaunchKernelGGL(k_smin_partials, dim3(1), dim3(1024), 0, s, partial, nblk);
-    hipLaunchKernelGGL(k_smin_final, dim3(nblk), dim3(256), 0, s, x, n, partial);
-    MG_HIP(hipGetLastError());
-    return MIGRID_OK;
-}
 
 int exclusive_scan(const int32_t* in, int64_t n, int32_t* out, int32_t* partial, bool write_total,
                    hipStream_t s)
@@ -467,9 +431,8 @@ int sort_keys(int64_t E, int64_t K, void* scratch, int32_t* perm, int32_t* offse
     const int64_t big = K > 256 * (nblk + 1) ? K : 256 * (nblk + 1);
     int32_t* partial = (int32_t*)take((size_t)scan_scratch_elems(big) * 4);
 
-    // offsets start as the marks array: every key "first seen at E"
-    MG_TRY(fill_u32(offsets, (uint32_t)E, K, s));
-    if (E > 0) {
+    if (E == 0) return fill_u32(offsets, 0u, K, s);
+    {
         int bits = 1;
         while (((int64_t)1 << bits) < K) bits++;  // keys are < K
         const int passes = (bits + 7) / 8;
@@ -490,11 +453,13 @@ int sort_keys(int64_t E, int64_t K, void* scratch, int32_t* perm, int32_t* offse
             vi = vo;
             vo = next_vo;
         }
-        // ki: the sorted keys
+        // ki: the sorted keys; partial: the chunks' first positions
+        const int64_t nchunk = (K + kScanTile - 1) / kScanTile;
+        MG_TRY(fill_u32((uint32_t*)partial, (uint32_t)E, nchunk + 1, s));
         const int mb = (int)((E + 255) / 256 < 8192 ? (E + 255) / 256 : 8192);
-        hipLaunchKernelGGL(k_marks, dim3(mb), dim3(256), 0, s, ki, E, offsets);
+        hipLaunchKernelGGL(k_chunk_starts, dim3(mb), dim3(256), 0, s, ki, E, partial);
+        hipLaunchKernelGGL(k_lower_bounds, dim3(nchunk), dim3(256), 0, s, ki, K, partial, offsets);
         MG_HIP(hipGetLastError());
-        MG_TRY(suffix_min(offsets, K, partial, s));
     }
     return MIGRID_OK;
 }

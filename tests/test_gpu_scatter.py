@@ -162,3 +162,19 @@ def test_bin_sort_clustered_and_empty():
     np.testing.assert_array_equal(offs.cpu().numpy(), offs_ref)
     perm0, offs0 = ctx.bin_sort(dev(np.zeros((0, 2))), 64, 8, 8)
     assert perm0.numel() == 0 and not offs0.cpu().numpy().any()
+
+
+def test_bin_sort_dense_keys():
+    """40000 visibilities on one key and a few hundred elsewhere: the chunk
+    holding that key has more elements than the per-chunk scan takes, so its
+    offsets come from the per-key binary search."""
+    N, S, H = 256, 16, 16
+    rng = np.random.default_rng(5)
+    dense = np.tile(np.array([[100.25, 37.5]]), (40000, 1))
+    spread = rng.uniform(0, N, (700, 2))
+    uv = np.concatenate([spread[:300], dense, spread[300:]])
+    perm_ref, offs_ref = O.bin_sort(uv, N, S, H)
+    ctx = migrid.Context()
+    perm, offs = ctx.bin_sort(dev(uv), N, S, H)
+    np.testing.assert_array_equal(perm.cpu().numpy(), perm_ref)
+    np.testing.assert_array_equal(offs.cpu().numpy(), offs_ref)
