@@ -1843,6 +1843,12 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         return fail(MIGRID_ENOTSUP, "tiled: %lld keys / %lld visibilities in one plan (limit 2^31)",
                     (long long)p.K, (long long)p.E);
     const bool rec64 = p.key.stage == kStageF64;
+    // the column kernel addresses records by unsigned 32-bit byte offsets
+    if (p.E * (int64_t)(rec64 ? sizeof(RecF64) : sizeof(RecF32)) >= ((int64_t)1 << 32))
+        return fail(MIGRID_ENOTSUP,
+                    "tiled: %lld visibilities in one plan exceed 2^32 bytes of %d-byte records; "
+                    "split the batch",
+                    (long long)p.E, rec64 ? (int)sizeof(RecF64) : (int)sizeof(RecF32));
     p.off_perm = off;
     off += align256((size_t)(p.E > 0 ? p.E : 1) * 4);
     p.off_offs = off;
