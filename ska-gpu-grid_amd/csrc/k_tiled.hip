@@ -1474,9 +1474,25 @@ __global__ __launch_bounds__(256) void k_convert_gcf(const ConvJob* __restrict__
     const ConvJob c = jobs[blockIdx.y];
     const TI* __restrict__ src = (const TI*)c.src;
     G* __restrict__ dst = (G*)c.dst;
+    // a table of support s_src embedded in one of s_dst: sub-kernel element
+    // (r, c) moves to (r + o, c + o), o = ceil(s_dst / 2) - ceil(s_src / 2),
+    // which keeps every tap on its pixel; the rest is zero
+    const int sd = c.s_dst, ss = c.s_src;
+    const int o = (sd - sd / 2) - (ss - ss / 2);
+    const int64_t per = (int64_t)sd * sd;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < c.n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = Stage<G>::make((double)src[2 * i], (double)src[2 * i + 1]);
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t e = i;
+        bool in = true;
+        if (sd != ss) {
+            const int64_t kk = i / per, rc = i - kk * per;
+            const int r = (int)(rc / sd) - o, cc = (int)(rc % sd) - o;
+            in = r >= 0 && r < ss && cc >= 0 && cc < ss;
+            e = (kk * ss + r) * ss + cc;
+        }
+        dst[i] = in ? Stage<G>::make((double)src[2 * e], (double)src[2 * e + 1])
+                    : Stage<G>::make(0.0, 0.0);
+    }
 }
 
 int launch_convert(const ConvJob* table, int n, int64_t max_elems, int stage, bool in64,
@@ -1492,9 +1508,15 @@ int launch_convert(const ConvJob* table, int n, int64_t max_elems, int stage, bo
         else
             hipLaunchKernelGGL((k_convert_gcf<float, Bf16x2>), grid, dim3(256), 0, s, table);
     } else if (stage == kStageF32) {
-        hipLaunchKernelGGL((k_convert_gcf<double, float2>), grid, dim3(256), 0, s, table);
+        if (in64)
+            hipLaunchKernelGGL((k_convert_gcf<double, float2>), grid, dim3(256), 0, s, table);
+        else
+            hipLaunchKernelGGL((k_convert_gcf<float, float2>), grid, dim3(256), 0, s, table);
     } else {
-        hipLaunchKernelGGL((k_convert_gcf<float, double2>), grid, dim3(256), 0, s, table);
+        if (in64)
+            hipLaunchKernelGGL((k_convert_gcf<double, double2>), grid, dim3(256), 0, s, table);
+        else
+            hipLaunchKernelGGL((k_convert_gcf<float, double2>), grid, dim3(256), 0, s, table);
     }
     MG_HIP(hipGetLastError());
     return MIGRID_OK;
@@ -1769,6 +1791,15 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
                           (t_edge + x.S - 2) / 16 + 1 <= kTapRows;
         const TileCfg tc = pick_tile(x.S, o.tile, taps);
         k.T = t_edge;  // the tile option; each job's geometry is in t.g
+        // The column kernel has constant-support code for S = 32 and 64; a
+        // table job of another support runs as the next of them, its table
+        // embedded in zeros (converted copy), with the same pixels per tap:
+        // the extra taps add exact zeros.  Measured at 4096^2 / 2M
+        // clustered, this is 1.5-2x faster than the general code.
+        const int sk = (tc.T == kColTx && k.gm == 0 && x.S != 32 && x.S != 64)
+                           ? (x.S < 32 ? 32 : 64)
+                           : x.S;
+        if (x.N + sk - 2 >= 65536) return fail(MIGRID_ENOTSUP, "tiled: N + S - 2 must be < 65536");
         int vt = kValPlain;
         if (x.debug)
             vt = kValDebug;
@@ -1783,9 +1814,9 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         }
         TiledJob& t = p.jobs[j];
         t.a = x;
-        t.bg = bin_geom(x.N, x.S, tc.H);
+        t.bg = bin_geom(x.N, sk, tc.H);
         t.g.N = x.N;
-        t.g.S = x.S;
+        t.g.S = sk;
         t.g.T = tc.T;
         t.g.Ty = tc.Ty;
         t.g.H = tc.H;
@@ -1805,12 +1836,12 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         t.off_tcnt = off;
         off += align256((size_t)ntiles * kSubKernels * 4);
         // a table in the staged type unless the caller's table already is one
-        t.convert = k.gm == 0 && !((k.stage == kStageF64 && x.in_type == MIGRID_F64) ||
-                                   (k.stage == kStageF32 && x.in_type == MIGRID_F32));
+        t.convert = k.gm == 0 && (sk != x.S || !((k.stage == kStageF64 && x.in_type == MIGRID_F64) ||
+                                                  (k.stage == kStageF32 && x.in_type == MIGRID_F32)));
         // jobs of one table (same source and support) share its conversion
         for (int i = 0; t.convert && i < j; i++) {
             const TiledJob& u = p.jobs[i];
-            if (u.convert && !u.conv_dup && u.a.gcf == x.gcf && u.a.S == x.S) {
+            if (u.convert && !u.conv_dup && u.a.gcf == x.gcf && u.a.S == x.S && u.g.S == sk) {
                 t.off_gtab = u.off_gtab;
                 t.conv_dup = true;
                 break;
@@ -1819,9 +1850,9 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
         if (t.convert && !t.conv_dup) {
             const size_t gsz = k.stage == kStageBf16 ? 4 : k.stage == kStageF32 ? 8 : 16;
             t.off_gtab = off;
-            off += align256((size_t)kSubKernels * x.S * x.S * gsz);
+            off += align256((size_t)kSubKernels * sk * sk * gsz);
             p.nconv++;
-            p.max_gcf = std::max<int64_t>(p.max_gcf, (int64_t)kSubKernels * x.S * x.S);
+            p.max_gcf = std::max<int64_t>(p.max_gcf, (int64_t)kSubKernels * sk * sk);
         }
         t.taps = taps;
         (t.taps ? item_cap_t : item_cap) += ntiles * kSubKernels;
@@ -1830,10 +1861,10 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
             smax_t = std::max(smax_t, x.S);
             smin_t = std::min(smin_t, x.S);
         } else {
-            smax_owner = std::max(smax_owner, x.S);
-            smin_owner = std::min(smin_owner, x.S);
+            smax_owner = std::max(smax_owner, sk);
+            smin_owner = std::min(smin_owner, sk);
         }
-        p.smax = std::max(p.smax, x.S);
+        p.smax = std::max(p.smax, sk);
     }
     p.owner = smax_owner > 0;
     p.smax_owner = smax_owner;
@@ -2023,16 +2054,18 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
         pj[j].k_base = t.k_base;
         pj[j].nkeys = t.bg.nkeys;
         pj[j].N = t.bg.N;
-        pj[j].S = t.a.S;
+        pj[j].S = t.g.S;
         pj[j].H = t.bg.H;
         pj[j].nb = t.bg.nb;
-        pj[j].pitch = t.g.T == kColTx ? col_pitch(t.a.S) : stage_pitch(t.a.S);
+        pj[j].pitch = t.g.T == kColTx ? col_pitch(t.g.S) : stage_pitch(t.g.S);
         pj[j].esize = p.key.stage == kStageF64 ? 16 : p.key.stage == kStageF32 ? 8 : 4;
         if (t.convert && !t.conv_dup) {
             ConvJob c;
             c.src = t.a.gcf;
             c.dst = arena + t.off_gtab;
-            c.n = (int64_t)kSubKernels * t.a.S * t.a.S;
+            c.n = (int64_t)kSubKernels * t.g.S * t.g.S;
+            c.s_src = t.a.S;
+            c.s_dst = t.g.S;
             cj.push_back(c);
         }
     }

@@ -150,7 +150,9 @@ struct PackJob {
 struct ConvJob {
     const void* src;
     void* dst;
-    int64_t n;  // complex elements
+    int64_t n;          // complex elements written (64 s_dst^2)
+    int32_t s_src;      // the caller's table support
+    int32_t s_dst;      // the staged table's (s_src, or the support it is embedded in)
 };
 
 // Device layout of one owner launch over n jobs.  The jobs' visibilities

@@ -155,3 +155,34 @@ def test_batch_of_one_support(S):
         for j, r in zip(jobs, refs):
             assert O.rel_errors(cplx(j["grid"]), r)[0] < 1e-12, (mode, j["N"])
     b.close()
+
+
+@pytest.mark.parametrize("arith,tol", [(migrid.ARITH_NATIVE, 1e-12), (migrid.ARITH_F32, 1e-5)])
+def test_batch_of_embedded_supports(arith, tol):
+    """Owner jobs of supports 20, 24, 40, 48 and 64 in one launch: the first
+    two run embedded in 32, the next two in 64, with one table shared by two
+    jobs; graph replay into NaN-filled grids."""
+    t = torch()
+    print(f"[batch embedded supports] arith={arith}", flush=True)
+    shapes = [(96, 20, 2000), (150, 24, 4000), (128, 40, 3000), (110, 48, 2500), (200, 64, 3000),
+              (90, 24, 1500)]
+    gcfs = {S: O.init_gcf(S) for S in (20, 24, 40, 48, 64)}
+    dg = {S: dev(g) for S, g in gcfs.items()}
+    jobs, refs = [], []
+    for i, (N, S, n) in enumerate(shapes):
+        uv, vals = O.gen_clustered(n, N, S, mean_r=N / 6, seed=70 + i)
+        refs.append(cplx_np(O.grid_zz(uv, vals, N, gcfs[S], S)))
+        jobs.append(dict(grid=t.empty((N, N, 2), dtype=t.float64, device="cuda"), uv=dev(uv),
+                         vals=dev(vals), gcf=dg[S], N=N, S=S))
+    ctx = migrid.Context()
+    b = migrid.Batch(ctx, jobs, arith=arith)
+    stream = t.cuda.Stream()
+    for mode in ("run", "graph"):
+        for j in jobs:
+            j["grid"].fill_(float("nan"))
+        t.cuda.synchronize()
+        b.run(stream) if mode == "run" else b.graph_launch(stream)
+        t.cuda.synchronize()
+        for j, r in zip(jobs, refs):
+            assert O.rel_errors(cplx(j["grid"]), r)[0] < tol, (mode, j["N"], j["S"])
+    b.close()
