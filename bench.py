@@ -461,7 +461,7 @@ def main_serve(args, torch, migrid):
     plain_vis = sum(n for _, _, n, _ in prefix)
     parts = [(pools[jid][2], pools[jid][3], n) for _, jid, n, _ in trace]
     out = {
-        "metric": "Mvis/s and p99 latency of the serving loop, Poisson 200 req/s",
+        "metric": f"Mvis/s and p99 latency of the serving loop, Poisson {args.rate:g} req/s",
         "value": total / wall / 1e6, "unit": "Mvis/s", "n_gpus": 1, "steps": len(trace),
         "warmup": 1, "ms_per_step": wall * 1e3 / len(trace), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",

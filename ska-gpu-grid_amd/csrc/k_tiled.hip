@@ -450,6 +450,7 @@ __host__ __device__ constexpr int stage_pitch(int S)
     return stage_rows(S) + ((8 - stage_rows(S) % 32) + 32) % 32;
 }
 constexpr int kStageMinDefault = 4;  // fewer visibilities for a k: read the table from HBM
+constexpr int kStageMinMixed = 2;    // the same for supports 32, 64 in a launch of mixed supports
 // LDS of a workgroup beside the kernel's static arrays (160 KB per CU)
 constexpr size_t kMaxLds = 160 * 1024 - 4096;
 
@@ -999,10 +1000,14 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner_col(
                                 ? jobs[job].tcount[(int64_t)tile * kSubKernels + lane]
                                 : 0;
             nz_mask = __ballot(cnt > 0);
-            // (supports 32 and 64 stage every non-empty sub-kernel unless a
-            // threshold is given; stage_min 0 is the default)
-            const bool all = GM != 0 || SC != 0 || ((S == 32 || S == 64) && stage_min == 0);
-            const int smin = stage_min > 0 ? stage_min : kStageMinDefault;
+            // (with the support a constant every non-empty sub-kernel is
+            // staged; in a launch of mixed supports, items of supports 32 and
+            // 64 read single visibilities' taps from memory by default, the
+            // autotuner's choice for the 64-job batch)
+            const bool all = GM != 0 || SC != 0;
+            const int smin = stage_min > 0 ? stage_min
+                             : (S == 32 || S == 64) ? kStageMinMixed
+                                                    : kStageMinDefault;
             st_mask = all ? nz_mask : __ballot(cnt >= smin && cnt > 0);
         }
         // sub-kernel rows this wave stages: w, w + NW, ... (one column per
