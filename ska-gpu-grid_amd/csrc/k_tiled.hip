@@ -1454,13 +1454,30 @@ __global__ __launch_bounds__(256) void k_reduce_tiles(const JobDesc* __restrict_
         TO* lo = (TO*)d.grid_lo;
         const bool fresh = d.fresh != 0;
         const int64_t NN = (int64_t)g.N * g.N;
+        // a pixel's (re, im) partials by one 16-byte load per slot, four slots
+        // in flight, added in slot order (the order fixes the result)
+        using V2 = typename Pair2<TA>::T;
+        const V2* __restrict__ p2 = (const V2*)part;
+        const int64_t s2 = slot / 2;
         for (int e = threadIdx.x; e < g.T * g.Ty; e += blockDim.x) {
             const int x = tx * g.T + e % g.T, y = ty * g.Ty + e / g.T;
             if (x >= g.N || y >= g.N) continue;
             TA re = 0, im = 0;
-            for (int i = 0; i < ri.n; i++) {
-                re += part[i * slot + 2 * e];
-                im += part[i * slot + 2 * e + 1];
+            int i = 0;
+            for (; i + 4 <= ri.n; i += 4) {
+                V2 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) v[u] = p2[(i + u) * s2 + e];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    re += v[u].x;
+                    im += v[u].y;
+                }
+            }
+            for (; i < ri.n; i++) {
+                const V2 v = p2[i * s2 + e];
+                re += v.x;
+                im += v.y;
             }
             const int64_t p = (int64_t)y * g.N + x;
             if (planar)
