@@ -242,3 +242,19 @@ def test_embedded_supports(S, N, strategy):
         g = grid.cpu().numpy()
         g = g[0] + 1j * g[1] if planar else g[..., 0] + 1j * g[..., 1]
         assert O.rel_errors(g, ref)[0] < tol, (arith, S)
+
+
+def test_record_bytes_limit():
+    """A tiled plan whose packed records would pass 2^32 bytes (the column
+    kernel's byte offsets are 32-bit) is refused before any work: 2^27
+    visibilities of 32-byte FP64 records."""
+    t = torch()
+    n = 1 << 27
+    uv = t.empty((n, 2), dtype=t.float64, device="cuda")
+    vals = t.empty((n, 2), dtype=t.float64, device="cuda")
+    grid = t.empty((64, 64, 2), dtype=t.float64, device="cuda")
+    gcf = dev(O.init_gcf(8))
+    with pytest.raises(migrid.MigridError) as e:
+        migrid.Context().grid(grid, uv, vals, gcf, 64, 8, strategy="gather")
+    assert "2^32 bytes" in str(e.value)
+    del uv, vals
