@@ -274,12 +274,16 @@ __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs,
                                                Item* __restrict__ items_o,
                                                QueueState* __restrict__ qs_o,
                                                Item* __restrict__ items_t,
-                                               QueueState* __restrict__ qs_t)
+                                               QueueState* __restrict__ qs_t,
+                                               Item* __restrict__ items_2,
+                                               QueueState* __restrict__ qs_2)
 {
     const int job = blockIdx.y;
-    // the owner kernel's queue, or the small-support kernel's
-    Item* __restrict__ items = jobs[job].taps ? items_t : items_o;
-    QueueState* __restrict__ qs = jobs[job].taps ? qs_t : qs_o;
+    // the owner kernel's queue, the small-support kernel's, or the second
+    // owner queue; split tiles of the ordered reduction are listed in the
+    // owner queue's counters whichever queue holds their items
+    Item* __restrict__ items = jobs[job].taps ? items_t : jobs[job].q2 ? items_2 : items_o;
+    QueueState* __restrict__ qs = jobs[job].taps ? qs_t : jobs[job].q2 ? qs_2 : qs_o;
     const int64_t target = jobs[job].taps ? target_t : target_o;
     const TileGeom g = jobs[job].g;
     const int32_t* __restrict__ tcount = jobs[job].tcount;
@@ -328,14 +332,14 @@ __global__ __launch_bounds__(256) void k_items(const JobDesc* __restrict__ jobs,
     if (lane == 0) {
         base = atomicAdd(&qs->nitems, n_items);
         if (det && n_items > 1) {  // list the split tile for the ordered reduction
-            const int rb = atomicAdd(&qs->nreduce, 1);
-            const int pb = atomicAdd(&qs->npartial, n_items);
+            const int rb = atomicAdd(&qs_o->nreduce, 1);
+            const int pb = atomicAdd(&qs_o->npartial, n_items);
             // the plan's capacities bound every tile's need (tiled_plan), so
             // this always fits; were it not to, the tile would flush with
             // atomics (correct, not bitwise reproducible), counted in
             // qs->overflow
             const bool fits = rb < jobs[job].rcap && pb + n_items <= jobs[job].pcap;
-            if (!fits) atomicAdd(&qs->overflow, 1);
+            if (!fits) atomicAdd(&qs_o->overflow, 1);
             if (rb < jobs[job].rcap) {
                 ReduceItem ri;
                 ri.job = job;
@@ -1785,7 +1789,7 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p = TiledPlan();
     p.jobs.resize(n);
     size_t off = 0;
-    int64_t item_cap = 0, item_cap_t = 0;
+    int64_t item_cap = 0, item_cap_t = 0, item_cap_2 = 0;
     int smax_owner = 0, smin_owner = 1 << 30, smax_t = 0, smin_t = 1 << 30;
     // small supports: one tap per lane into an LDS tile (atomic strategy)
     const int tap_smax = o.tap_smax == 0 ? kTapMaxS : std::min(o.tap_smax, kTapMaxS);
@@ -1892,6 +1896,34 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.smax_owner = smax_owner;
     p.s_common = smin_owner == smax_owner ? smax_owner : 0;
     p.s_common_t = smin_t == smax_t ? smax_t : 0;
+    // Column-kernel jobs of support 32 beside jobs of other supports get a
+    // queue and a launch of their own, so both sets run the instantiations
+    // with the support a constant (the general one, deciding per item, was
+    // 35-40% slower on an S = 64 job).
+    if (p.owner && p.s_common == 0 && p.key.T == 64) {
+        int hi = 0, lo = 1 << 30;
+        bool has32 = false;
+        for (const TiledJob& t : p.jobs) {
+            if (t.taps) continue;
+            if (t.g.S == 32) {
+                has32 = true;
+            } else {
+                hi = std::max(hi, t.g.S);
+                lo = std::min(lo, t.g.S);
+            }
+        }
+        if (has32 && hi > 0) {
+            for (TiledJob& t : p.jobs) {
+                if (t.taps || t.g.S != 32) continue;
+                t.q2 = true;
+                const int64_t nt = (int64_t)t.g.ntx * t.g.nty * kSubKernels;
+                item_cap -= nt;
+                item_cap_2 += nt;
+            }
+            p.owner2 = true;
+            p.s_common = hi == lo ? hi : 0;
+        }
+    }
     if (p.K >= ((int64_t)1 << 31) || p.E >= ((int64_t)1 << 31))
         return fail(MIGRID_ENOTSUP, "tiled: %lld keys / %lld visibilities in one plan (limit 2^31)",
                     (long long)p.K, (long long)p.E);
@@ -1920,9 +1952,14 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     off += align256((size_t)(item_cap > 0 ? item_cap : 1) * sizeof(Item));
     p.off_items_t = off;
     off += align256((size_t)(item_cap_t > 0 ? item_cap_t : 1) * sizeof(Item));
-    p.off_qs = off;  // the owner kernel's queue, then the small-support kernel's
+    p.off_items_2 = off;
+    off += align256((size_t)(item_cap_2 > 0 ? item_cap_2 : 1) * sizeof(Item));
+    // the owner kernel's queue, the small-support kernel's, the second owner
+    // queue's (support-32 jobs beside others)
+    p.off_qs = off;
     p.off_qs_t = off + sizeof(QueueState);
-    off += align256(2 * sizeof(QueueState));
+    p.off_qs_2 = off + 2 * sizeof(QueueState);
+    off += align256(3 * sizeof(QueueState));
     // jobs that write one grid flush atomically, into a grid cleared beforehand
     {
         std::vector<const void*> grids;
@@ -1973,6 +2010,7 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.bin_bytes = bin_scratch_bytes(p.E, p.K);
     p.item_cap = item_cap;
     p.item_cap_t = item_cap_t;
+    p.item_cap_2 = item_cap_2;
     p.stage_min = o.stage_min;
     p.wg_per_cu = o.wg_per_cu;
     const size_t gsz = p.key.stage == kStageBf16 ? 4 : p.key.stage == kStageF32 ? 8 : 16;
@@ -2002,6 +2040,7 @@ void tiled_descs(const TiledPlan& p, char* arena, std::vector<JobDesc>& d)
         d[j].shared = p.shared ? 1 : 0;
         d[j].det = p.det ? 1 : 0;
         d[j].taps = t.taps ? 1 : 0;
+        d[j].q2 = t.q2 ? 1 : 0;
         d[j].pcap = (int32_t)p.partial_cap;
         d[j].rcap = (int32_t)p.reduce_cap;
         d[j].partials = p.det ? (void*)(arena + p.off_partials) : nullptr;
@@ -2061,7 +2100,7 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
     Item* items = (Item*)(arena + p.off_items);
     int32_t* perm = (int32_t*)(arena + p.off_perm);
     int32_t* offs = (int32_t*)(arena + p.off_offs);
-    MG_TRY(fill_u32(qs, 0u, 2 * sizeof(QueueState) / 4, s));  // both queues
+    MG_TRY(fill_u32(qs, 0u, 3 * sizeof(QueueState) / 4, s));  // the three queues
     // job tables
     std::vector<PackJob> pj(n);
     std::vector<ConvJob> cj;
@@ -2150,12 +2189,14 @@ int tiled_prepare(const TiledPlan& p, char* arena, void* bin_scratch, hipStream_
     const dim3 ig((unsigned)((max_tiles + 3) / 4), n);
     Item* items_t = (Item*)(arena + p.off_items_t);
     QueueState* qs_t = (QueueState*)(arena + p.off_qs_t);
+    Item* items_2 = (Item*)(arena + p.off_items_2);
+    QueueState* qs_2 = (QueueState*)(arena + p.off_qs_2);
     if (p.key.out_type == MIGRID_F64)
         hipLaunchKernelGGL(k_items<double>, ig, dim3(256), 0, s, ddesc, p.target, p.target_t,
-                           p.key.planar ? 1 : 0, items, qs, items_t, qs_t);
+                           p.key.planar ? 1 : 0, items, qs, items_t, qs_t, items_2, qs_2);
     else
         hipLaunchKernelGGL(k_items<float>, ig, dim3(256), 0, s, ddesc, p.target, p.target_t,
-                           p.key.planar ? 1 : 0, items, qs, items_t, qs_t);
+                           p.key.planar ? 1 : 0, items, qs, items_t, qs_t, items_2, qs_2);
     MG_HIP(hipGetLastError());
     return MIGRID_OK;
 }
@@ -2180,6 +2221,13 @@ int tiled_execute(const TiledPlan& p, char* arena, hipStream_t s)
     }
     L.sc = p.s_common;
     if (p.owner) MG_TRY(dispatch_owner(p.key, L, s));
+    if (p.owner2) {
+        OwnerLaunch L2 = L;
+        L2.items = (const Item*)(arena + p.off_items_2);
+        L2.qs = (QueueState*)(arena + p.off_qs_2);
+        L2.sc = 32;
+        MG_TRY(dispatch_owner(p.key, L2, s));
+    }
     if (p.det) {  // the split tiles, summed in item order
         const int nb = (int)std::min<int64_t>(p.reduce_cap, 2048);
         const int planar = p.key.planar ? 1 : 0;

@@ -82,6 +82,7 @@ struct JobDesc {
     // so the grid is the same bit for bit on every run and no atomics are used
     int32_t det;
     int32_t taps;        // gridded by the small-support kernel (its own item queue)
+    int32_t q2;          // a support-32 job beside other supports: the second owner queue
     int32_t pcap, rcap;  // partial-tile slots and reduce entries allocated
     void* partials;      // pcap tiles of T x T complex accumulators
     ReduceItem* reduce;  // rcap entries
@@ -126,6 +127,7 @@ struct TiledJob {
     bool conv_dup = false;  // ... by an earlier job of the same table (shared copy)
     size_t off_gtab = 0;
     bool taps = false;  // small support: gridded by k_taps
+    bool q2 = false;    // support 32 beside other supports: second owner queue and launch
 };
 
 // Keys and records of one job (device table, blockIdx.y of k_keyrec and
@@ -194,6 +196,11 @@ struct TiledPlan {
     size_t lds_taps = 0;
     size_t off_items_t = 0, off_qs_t = 0;
     int64_t item_cap_t = 0;
+    // support-32 column-kernel jobs beside jobs of other supports: their own
+    // queue, gridded by a second owner launch (the S = 32 instantiation)
+    bool owner2 = false;
+    size_t off_items_2 = 0, off_qs_2 = 0;
+    int64_t item_cap_2 = 0;
 };
 
 struct OwnerLaunch {
