@@ -914,7 +914,9 @@ __device__ __forceinline__ void static_for(F&& f)
 }
 
 // SC: every job's support as a compile-time constant (0: read per job), which
-// makes the bin rows' group ranges and the LDS offsets constants.
+// makes the bin rows' group ranges and the LDS offsets constants; kScBoth:
+// every job at 64 or 32, the constant chosen per item.
+constexpr int kScBoth = -1;
 template <typename G, typename TA, typename R, typename TI, typename TO, bool PLANAR, int GM,
           int SC>
 __global__ __launch_bounds__(1024)
@@ -959,277 +961,290 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner_col(
         const double sigma = jobs[job].sigma, beta = jobs[job].beta;
         const bool atomic = jobs[job].split[tile] != 0;
         const bool fresh = jobs[job].fresh != 0;
-        const int S = SC ? SC : g.S;
-        const int L = col_lead(S);
-        const int pitch = col_pitch(S);
-        const int64_t NN = (int64_t)g.N * g.N;
-        if (S != staged_s) {
-            // a new layout: clear the old interior, then (constant-tap form)
-            // write the all-ones kernel once
-            __syncthreads();
-            if (staged_s > 0) {
-                const int ol = col_lead(staged_s), op = col_pitch(staged_s);
-                for (int e = tid; e < staged_s * staged_s; e += NT)
-                    sg[(e / staged_s + ol) * op + e % staged_s + ol] = Stage<G>::make(0.0, 0.0);
+        // the item's work with its support SX a compile-time constant (0: read
+        // from the job); SC = kScBoth picks 64 or 32 per item
+        auto body = [&](auto scx) {
+            constexpr int SX = decltype(scx)::value;
+            const int S = SX ? SX : g.S;
+            const int L = col_lead(S);
+            const int pitch = col_pitch(S);
+            const int64_t NN = (int64_t)g.N * g.N;
+            if (S != staged_s) {
+                // a new layout: clear the old interior, then (constant-tap form)
+                // write the all-ones kernel once
                 __syncthreads();
+                if (staged_s > 0) {
+                    const int ol = col_lead(staged_s), op = col_pitch(staged_s);
+                    for (int e = tid; e < staged_s * staged_s; e += NT)
+                        sg[(e / staged_s + ol) * op + e % staged_s + ol] = Stage<G>::make(0.0, 0.0);
+                    __syncthreads();
+                }
+                if (GM == 1)
+                    for (int e = tid; e < S * S; e += NT)
+                        sg[(e / S + L) * pitch + e % S + L] = Stage<G>::make(1.0, 0.0);
+                staged_s = S;
             }
-            if (GM == 1)
-                for (int e = tid; e < S * S; e += NT)
-                    sg[(e / S + L) * pitch + e % S + L] = Stage<G>::make(1.0, 0.0);
-            staged_s = S;
-        }
-        __syncthreads();
-        const int tx = tile % g.ntx, ty = tile / g.ntx;
-        const int X0 = tx * kColTx, Y0 = ty * kColTy;
-        const int Xw = X0 + 8 * w, Yw = Y0;
-        const int nb = g.nb;
-        // this wave's bins: columns bxl .. bxl + ncx - 1, rows byl .. byl + nry - 1
-        // (fewer at the grid's far edge; none for a block past it, whose wave
-        // still takes part in the workgroup's barriers)
-        const int bxl = Xw / kColH, byl = Yw / kColH;
-        const bool on = Xw < g.N && Yw < g.N;
-        const int ncx = on ? min(nb - bxl, col_ncols(S)) : 0;
-        const int nry = on && ncx > 0 ? min(nb - byl, col_nrows(S)) : 0;
-        const int D = col_depth(S);
+            __syncthreads();
+            const int tx = tile % g.ntx, ty = tile / g.ntx;
+            const int X0 = tx * kColTx, Y0 = ty * kColTy;
+            const int Xw = X0 + 8 * w, Yw = Y0;
+            const int nb = g.nb;
+            // this wave's bins: columns bxl .. bxl + ncx - 1, rows byl .. byl + nry - 1
+            // (fewer at the grid's far edge; none for a block past it, whose wave
+            // still takes part in the workgroup's barriers)
+            const int bxl = Xw / kColH, byl = Yw / kColH;
+            const bool on = Xw < g.N && Yw < g.N;
+            const int ncx = on ? min(nb - bxl, col_ncols(S)) : 0;
+            const int nry = on && ncx > 0 ? min(nb - byl, col_nrows(S)) : 0;
+            const int D = col_depth(S);
 
-        Cplx<TA> acc[NG];
-#pragma unroll
-        for (int i = 0; i < NG; i++) acc[i].re = acc[i].im = (TA)0;
+            Cplx<TA> acc[NG];
+    #pragma unroll
+            for (int i = 0; i < NG; i++) acc[i].re = acc[i].im = (TA)0;
 
-        // lane k counts the workgroup's visibilities for sub-kernel k; the
-        // ballots give the non-empty and the staged sub-kernels of this item
-        uint64_t nz_mask, st_mask;
-        {
-            const int cnt = (lane >= klo && lane <= khi)
-                                ? jobs[job].tcount[(int64_t)tile * kSubKernels + lane]
-                                : 0;
-            nz_mask = __ballot(cnt > 0);
-            // (with the support a constant every non-empty sub-kernel is
-            // staged; in a launch of mixed supports, items of supports 32 and
-            // 64 read single visibilities' taps from memory by default, the
-            // autotuner's choice for the 64-job batch)
-            const bool all = GM != 0 || SC != 0;
-            const int smin = stage_min > 0 ? stage_min
-                             : (S == 32 || S == 64) ? kStageMinMixed
-                                                    : kStageMinDefault;
-            st_mask = all ? nz_mask : __ballot(cnt >= smin && cnt > 0);
-        }
-        // sub-kernel rows this wave stages: w, w + NW, ... (one column per
-        // lane); the next staged sub-kernel is fetched into registers while
-        // the current one is in use
-        constexpr bool PF = GM == 0;
-        constexpr int PR = PF ? (64 + NW - 1) / NW : 1;
-        G pf[PR];
-        auto fetch_rows = [&](int kk) {
-            GlobalPtr<G> src = (GlobalPtr<G>)(uintptr_t)(gtab + (int64_t)kk * S * S);
-#pragma unroll
-            for (int p = 0; p < PR; p++) {
-                const int r = w + p * NW;
-                if (r < S && lane < S) pf[p] = gload<G>(src, r * S + lane);
-            }
-        };
-        if (PF && st_mask) fetch_rows(__builtin_ctzll(st_mask));
-
-        // Per lane: the staged element of group 0 for q = 0, in bytes; a
-        // visibility shifts it back by its record offset (q_y P + q_x
-        // elements, in bytes), group i lies 8 i P elements further.
-        const char* lb = (const char*)sg +
-                         (size_t)((Yw + ly + S - 1 + L) * pitch + Xw + lx + S - 1 + L) * sizeof(G);
-
-        // CSR bounds of the wave's bin rows for one sub-kernel, gathered by
-        // one vector load (lane 2t: start of row t, lane 2t + 1: its end) and
-        // fetched one sub-kernel ahead; row t's key is key0 + t nb
-        GlobalPtr<int32_t> offs_v = (GlobalPtr<int32_t>)(uintptr_t)jobs[job].offs;
-        auto load_bounds = [&](int kk) -> int {
-            int v = 0;
-            if (lane < 2 * nry) {
-                const int key = (kk * nb + byl + (lane >> 1)) * nb + bxl;
-                v = offs_v[(lane & 1) ? key + ncx : key];
-            }
-            return v;
-        };
-        int bnd_next = nz_mask ? load_bounds(__builtin_ctzll(nz_mask)) : 0;
-
-        for (uint64_t todo = nz_mask; todo; todo &= todo - 1) {
-            const int k = __builtin_ctzll(todo);
-            const bool staged = (st_mask >> k) & 1;
-            // first key of the wave's bin rows: row t at key0 + t nb
-            const int key0 = (k * nb + byl) * nb + bxl;
-            const int bnd = bnd_next;
+            // lane k counts the workgroup's visibilities for sub-kernel k; the
+            // ballots give the non-empty and the staged sub-kernels of this item
+            uint64_t nz_mask, st_mask;
             {
-                const uint64_t rest = todo & (todo - 1);
-                if (rest) bnd_next = load_bounds(__builtin_ctzll(rest));
+                const int cnt = (lane >= klo && lane <= khi)
+                                    ? jobs[job].tcount[(int64_t)tile * kSubKernels + lane]
+                                    : 0;
+                nz_mask = __ballot(cnt > 0);
+                // (with the support a constant every non-empty sub-kernel is
+                // staged; in a launch of mixed supports, items of supports 32 and
+                // 64 read single visibilities' taps from memory by default, the
+                // autotuner's choice for the 64-job batch)
+                const bool all = GM != 0 || SX != 0;
+                const int smin = stage_min > 0 ? stage_min
+                                 : (S == 32 || S == 64) ? kStageMinMixed
+                                                        : kStageMinDefault;
+                st_mask = all ? nz_mask : __ballot(cnt >= smin && cnt > 0);
             }
-            if (staged && GM != 1) {
-                __syncthreads();  // previous sub-kernel no longer read
-                if (PF) {
-#pragma unroll
-                    for (int p = 0; p < PR; p++) {
-                        const int r = w + p * NW;
-                        if (r < S && lane < S) sg[(r + L) * pitch + lane + L] = pf[p];
-                    }
-                    const uint64_t later = st_mask & ~((2ull << k) - 1);
-                    if (later) fetch_rows(__builtin_ctzll(later));
-                } else {
-                    // separable analytic GCF: G[r][c] = f(dy_r) * f(dx_c),
-                    // f(d) = exp(-d^2/(2 sigma^2)) * exp(i beta d^2)
-                    const int sx = k % kOversample, sy = k / kOversample;
-                    for (int e = tid; e < 2 * S; e += NT) {
-                        const int axis = e / S, t = e % S;
-                        const double d = (double)(t - S / 2 + 1) - (axis ? sy : sx) / 8.0;
-                        const double q = d * d;
-                        const double amp = exp(-q / (2.0 * sigma * sigma));
-                        double sn, cs;
-                        sincos(beta * q, &sn, &cs);
-                        s_fac[axis][t].re = amp * cs;
-                        s_fac[axis][t].im = amp * sn;
+            // sub-kernel rows this wave stages: w, w + NW, ... (one column per
+            // lane); the next staged sub-kernel is fetched into registers while
+            // the current one is in use
+            constexpr bool PF = GM == 0;
+            constexpr int PR = PF ? (64 + NW - 1) / NW : 1;
+            G pf[PR];
+            auto fetch_rows = [&](int kk) {
+                GlobalPtr<G> src = (GlobalPtr<G>)(uintptr_t)(gtab + (int64_t)kk * S * S);
+    #pragma unroll
+                for (int p = 0; p < PR; p++) {
+                    const int r = w + p * NW;
+                    if (r < S && lane < S) pf[p] = gload<G>(src, r * S + lane);
+                }
+            };
+            if (PF && st_mask) fetch_rows(__builtin_ctzll(st_mask));
+
+            // Per lane: the staged element of group 0 for q = 0, in bytes; a
+            // visibility shifts it back by its record offset (q_y P + q_x
+            // elements, in bytes), group i lies 8 i P elements further.
+            const char* lb = (const char*)sg +
+                             (size_t)((Yw + ly + S - 1 + L) * pitch + Xw + lx + S - 1 + L) * sizeof(G);
+
+            // CSR bounds of the wave's bin rows for one sub-kernel, gathered by
+            // one vector load (lane 2t: start of row t, lane 2t + 1: its end) and
+            // fetched one sub-kernel ahead; row t's key is key0 + t nb
+            GlobalPtr<int32_t> offs_v = (GlobalPtr<int32_t>)(uintptr_t)jobs[job].offs;
+            auto load_bounds = [&](int kk) -> int {
+                int v = 0;
+                if (lane < 2 * nry) {
+                    const int key = (kk * nb + byl + (lane >> 1)) * nb + bxl;
+                    v = offs_v[(lane & 1) ? key + ncx : key];
+                }
+                return v;
+            };
+            int bnd_next = nz_mask ? load_bounds(__builtin_ctzll(nz_mask)) : 0;
+
+            for (uint64_t todo = nz_mask; todo; todo &= todo - 1) {
+                const int k = __builtin_ctzll(todo);
+                const bool staged = (st_mask >> k) & 1;
+                // first key of the wave's bin rows: row t at key0 + t nb
+                const int key0 = (k * nb + byl) * nb + bxl;
+                const int bnd = bnd_next;
+                {
+                    const uint64_t rest = todo & (todo - 1);
+                    if (rest) bnd_next = load_bounds(__builtin_ctzll(rest));
+                }
+                if (staged && GM != 1) {
+                    __syncthreads();  // previous sub-kernel no longer read
+                    if (PF) {
+    #pragma unroll
+                        for (int p = 0; p < PR; p++) {
+                            const int r = w + p * NW;
+                            if (r < S && lane < S) sg[(r + L) * pitch + lane + L] = pf[p];
+                        }
+                        const uint64_t later = st_mask & ~((2ull << k) - 1);
+                        if (later) fetch_rows(__builtin_ctzll(later));
+                    } else {
+                        // separable analytic GCF: G[r][c] = f(dy_r) * f(dx_c),
+                        // f(d) = exp(-d^2/(2 sigma^2)) * exp(i beta d^2)
+                        const int sx = k % kOversample, sy = k / kOversample;
+                        for (int e = tid; e < 2 * S; e += NT) {
+                            const int axis = e / S, t = e % S;
+                            const double d = (double)(t - S / 2 + 1) - (axis ? sy : sx) / 8.0;
+                            const double q = d * d;
+                            const double amp = exp(-q / (2.0 * sigma * sigma));
+                            double sn, cs;
+                            sincos(beta * q, &sn, &cs);
+                            s_fac[axis][t].re = amp * cs;
+                            s_fac[axis][t].im = amp * sn;
+                        }
+                        __syncthreads();
+                        for (int e = tid; e < S * S; e += NT) {
+                            const int r = e / S, c = e % S;
+                            const Cplx<double> a = s_fac[1][r], b = s_fac[0][c];
+                            sg[(r + L) * pitch + c + L] =
+                                Stage<G>::make(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+                        }
                     }
                     __syncthreads();
-                    for (int e = tid; e < S * S; e += NT) {
-                        const int r = e / S, c = e % S;
-                        const Cplx<double> a = s_fac[1][r], b = s_fac[0][c];
-                        sg[(r + L) * pitch + c + L] =
-                            Stage<G>::make(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
-                    }
                 }
-                __syncthreads();
-            }
-            // the support a constant SX: each bin row's groups and the group
-            // offsets are constants
-            auto rows_const = [&](auto sx) {
-                constexpr int SX = decltype(sx)::value;
-                constexpr int NR = col_nrows(SX), DC = col_depth(SX), PC = col_pitch(SX);
-                constexpr uint32_t RS = sizeof(R);
-                ConstPtr<char> rbase = (ConstPtr<char>)rec;
-                // the bounds as byte offsets into the records, and a mask
-                // of the non-empty rows (bit 2t: row t), so an empty row
-                // costs one bit test
-                const uint32_t bb = (uint32_t)bnd * RS;
-                const uint32_t bb_next =
-                    (uint32_t)__builtin_amdgcn_mov_dpp((int)bb, 0xB1, 0xF, 0xF, false);
-                const uint64_t rows_on = __ballot(!(lane & 1) && bb < bb_next);
-                static_for<NR>([&](auto tc) {
-                    constexpr int t = decltype(tc)::value;
-                    constexpr int A = t - DC > 0 ? t - DC : 0;
-                    constexpr int B = t < NG - 1 ? t : NG - 1;
-                    if (!((rows_on >> (2 * t)) & 1)) return;
-                    uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)bb, 2 * t);
-                    const uint32_t oe = (uint32_t)__builtin_amdgcn_readlane((int)bb, 2 * t + 1);
-                    // the taps of one visibility
-                    auto one = [&](uint32_t off, V vr, V vi) {
-                        const G* b = (const G*)(lb - off);
-                        P gr[NG], gi[NG];
-#pragma unroll
-                        for (int i = A; i <= B; i++) Stage<G>::get(b[8 * i * PC], gr[i], gi[i]);
-#pragma unroll
-                        for (int i = A; i <= B; i++)
-                            mac1<P, TA>(acc[i], (P)vr, (P)vi, gr[i], gi[i]);
-                    };
-                    // pairs of visibilities from one scalar record load,
-                    // then the row's odd last one alone
-                    const uint32_t oe1 = oe - RS;
-                    for (; o < oe1; o += 2 * RS) {
-                        uint32_t off[2];
-                        V vr[2], vi[2];
-                        pair_at<R>(rbase, o, off, vr, vi);
-                        one(off[0], vr[0], vi[0]);
-                        // rows of many groups: the second visibility's
-                        // taps are read after the first one's products,
-                        // so one visibility's taps are live at a time
-                        if constexpr (B - A + 1 > 4) __builtin_amdgcn_sched_barrier(0);
-                        one(off[1], vr[1], vi[1]);
+                // the support a constant SX: each bin row's groups and the group
+                // offsets are constants
+                auto rows_const = [&](auto sx) {
+                    constexpr int SX = decltype(sx)::value;
+                    constexpr int NR = col_nrows(SX), DC = col_depth(SX), PC = col_pitch(SX);
+                    constexpr uint32_t RS = sizeof(R);
+                    ConstPtr<char> rbase = (ConstPtr<char>)rec;
+                    // the bounds as byte offsets into the records, and a mask
+                    // of the non-empty rows (bit 2t: row t), so an empty row
+                    // costs one bit test
+                    const uint32_t bb = (uint32_t)bnd * RS;
+                    const uint32_t bb_next =
+                        (uint32_t)__builtin_amdgcn_mov_dpp((int)bb, 0xB1, 0xF, 0xF, false);
+                    const uint64_t rows_on = __ballot(!(lane & 1) && bb < bb_next);
+                    static_for<NR>([&](auto tc) {
+                        constexpr int t = decltype(tc)::value;
+                        constexpr int A = t - DC > 0 ? t - DC : 0;
+                        constexpr int B = t < NG - 1 ? t : NG - 1;
+                        if (!((rows_on >> (2 * t)) & 1)) return;
+                        uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)bb, 2 * t);
+                        const uint32_t oe = (uint32_t)__builtin_amdgcn_readlane((int)bb, 2 * t + 1);
+                        // the taps of one visibility
+                        auto one = [&](uint32_t off, V vr, V vi) {
+                            const G* b = (const G*)(lb - off);
+                            P gr[NG], gi[NG];
+    #pragma unroll
+                            for (int i = A; i <= B; i++) Stage<G>::get(b[8 * i * PC], gr[i], gi[i]);
+    #pragma unroll
+                            for (int i = A; i <= B; i++)
+                                mac1<P, TA>(acc[i], (P)vr, (P)vi, gr[i], gi[i]);
+                        };
+                        // pairs of visibilities from one scalar record load,
+                        // then the row's odd last one alone
+                        const uint32_t oe1 = oe - RS;
+                        for (; o < oe1; o += 2 * RS) {
+                            uint32_t off[2];
+                            V vr[2], vi[2];
+                            pair_at<R>(rbase, o, off, vr, vi);
+                            one(off[0], vr[0], vi[0]);
+                            // rows of many groups: the second visibility's
+                            // taps are read after the first one's products,
+                            // so one visibility's taps are live at a time
+                            if constexpr (B - A + 1 > 4) __builtin_amdgcn_sched_barrier(0);
+                            one(off[1], vr[1], vi[1]);
+                        }
+                        if (o == oe1) {
+                            uint32_t off;
+                            V vr, vi;
+                            rec_at<R>(rbase, o, off, vr, vi);
+                            one(off, vr, vi);
+                        }
+                    });
+                };
+                if (staged) {
+                    if constexpr (SX != 0) {
+                        rows_const(std::integral_constant<int, SX>{});
+                    } else if (S == 64) {
+                        // jobs of mixed supports: the common ones take the
+                        // constant-support rows, decided once per item
+                        rows_const(std::integral_constant<int, 64>{});
+                    } else if (S == 32) {
+                        rows_const(std::integral_constant<int, 32>{});
+                    } else {
+                        // any support: the groups of bin row t are tested per
+                        // visibility (wave-uniform branches)
+                        for (int t = 0; t < nry; t++) {
+                            const int A = max(0, t - D), B = min(NG - 1, t);
+                            const int r0 = offs[key0 + t * nb], r1 = offs[key0 + t * nb + ncx];
+                            for (int r = r0; r < r1; r++) {
+                                const G* b = (const G*)(lb - rec[r].off);
+                                const P vr = (P)rec[r].vr, vi = (P)rec[r].vi;
+    #pragma unroll
+                                for (int i = 0; i < NG; i++) {
+                                    if (i < A || i > B) continue;
+                                    P gr, gi;
+                                    Stage<G>::get(b[8 * i * pitch], gr, gi);
+                                    mac1<P, TA>(acc[i], vr, vi, gr, gi);
+                                }
+                            }
+                        }
                     }
-                    if (o == oe1) {
-                        uint32_t off;
-                        V vr, vi;
-                        rec_at<R>(rbase, o, off, vr, vi);
-                        one(off, vr, vi);
-                    }
-                });
-            };
-            if (staged) {
-                if constexpr (SC != 0) {
-                    rows_const(std::integral_constant<int, SC>{});
-                } else if (S == 64) {
-                    // jobs of mixed supports: the common ones take the
-                    // constant-support rows, decided once per item
-                    rows_const(std::integral_constant<int, 64>{});
-                } else if (S == 32) {
-                    rows_const(std::integral_constant<int, 32>{});
-                } else {
-                    // any support: the groups of bin row t are tested per
-                    // visibility (wave-uniform branches)
+                } else if constexpr (SX == 0) {
+                    // a sparse sub-kernel read from the table in memory (any
+                    // support; with the support a constant every sub-kernel is
+                    // staged, and the registers this path would hold go to the
+                    // staged loop)
+                    GlobalPtr<G> gk = (GlobalPtr<G>)(uintptr_t)(gtab + (int64_t)k * S * S);
                     for (int t = 0; t < nry; t++) {
                         const int A = max(0, t - D), B = min(NG - 1, t);
                         const int r0 = offs[key0 + t * nb], r1 = offs[key0 + t * nb + ncx];
                         for (int r = r0; r < r1; r++) {
-                            const G* b = (const G*)(lb - rec[r].off);
+                            const uint32_t q = rec[r].q;
+                            const int c = S - 1 - ((int)(q & 0xffffu) - Xw) + lx;
+                            const int rr0 = S - 1 - ((int)(q >> 16) - Yw) + ly;
                             const P vr = (P)rec[r].vr, vi = (P)rec[r].vi;
-#pragma unroll
+                            // groups in fours, 32-bit element offsets: few taps
+                            // and addresses live at a time
+    #pragma unroll
                             for (int i = 0; i < NG; i++) {
                                 if (i < A || i > B) continue;
-                                P gr, gi;
-                                Stage<G>::get(b[8 * i * pitch], gr, gi);
+                                const int rr = rr0 + 8 * i;
+                                P gr = (P)0, gi = (P)0;
+                                if ((unsigned)c < (unsigned)S && (unsigned)rr < (unsigned)S)
+                                    Stage<G>::get(gload<G>(gk, rr * S + c), gr, gi);
                                 mac1<P, TA>(acc[i], vr, vi, gr, gi);
+                                if (i % 4 == 3) __builtin_amdgcn_sched_barrier(0);
                             }
                         }
                     }
                 }
-            } else if constexpr (SC == 0) {
-                // a sparse sub-kernel read from the table in memory (any
-                // support; with the support a constant every sub-kernel is
-                // staged, and the registers this path would hold go to the
-                // staged loop)
-                GlobalPtr<G> gk = (GlobalPtr<G>)(uintptr_t)(gtab + (int64_t)k * S * S);
-                for (int t = 0; t < nry; t++) {
-                    const int A = max(0, t - D), B = min(NG - 1, t);
-                    const int r0 = offs[key0 + t * nb], r1 = offs[key0 + t * nb + ncx];
-                    for (int r = r0; r < r1; r++) {
-                        const uint32_t q = rec[r].q;
-                        const int c = S - 1 - ((int)(q & 0xffffu) - Xw) + lx;
-                        const int rr0 = S - 1 - ((int)(q >> 16) - Yw) + ly;
-                        const P vr = (P)rec[r].vr, vi = (P)rec[r].vi;
-                        // groups in fours, 32-bit element offsets: few taps
-                        // and addresses live at a time
-#pragma unroll
-                        for (int i = 0; i < NG; i++) {
-                            if (i < A || i > B) continue;
-                            const int rr = rr0 + 8 * i;
-                            P gr = (P)0, gi = (P)0;
-                            if ((unsigned)c < (unsigned)S && (unsigned)rr < (unsigned)S)
-                                Stage<G>::get(gload<G>(gk, rr * S + c), gr, gi);
-                            mac1<P, TA>(acc[i], vr, vi, gr, gi);
-                            if (i % 4 == 3) __builtin_amdgcn_sched_barrier(0);
-                        }
-                    }
-                }
             }
-        }
-        // flush: a split tile of the ordered-reduction mode writes its
-        // partial tile (128 x 64 accumulators) to its slot
-        if (pslot >= 0) {
-            TA* part = (TA*)jobs[job].partials + (int64_t)pslot * kColTx * kColTy * 2;
-#pragma unroll
+            // flush: a split tile of the ordered-reduction mode writes its
+            // partial tile (128 x 64 accumulators) to its slot
+            if (pslot >= 0) {
+                TA* part = (TA*)jobs[job].partials + (int64_t)pslot * kColTx * kColTy * 2;
+    #pragma unroll
+                for (int i = 0; i < NG; i++) {
+                    const int e = (Yw - Y0 + ly + 8 * i) * kColTx + (Xw - X0 + lx);
+                    part[2 * e] = acc[i].re;
+                    part[2 * e + 1] = acc[i].im;
+                }
+                __syncthreads();  // s_item reuse
+                return;
+            }
+            const int x = Xw + lx;
+    #pragma unroll
             for (int i = 0; i < NG; i++) {
-                const int e = (Yw - Y0 + ly + 8 * i) * kColTx + (Xw - X0 + lx);
-                part[2 * e] = acc[i].re;
-                part[2 * e + 1] = acc[i].im;
+                const int y = Yw + ly + 8 * i;
+                if (x >= g.N || y >= g.N) continue;
+                const int64_t p = (int64_t)y * g.N + x;
+                if (PLANAR)
+                    flush_px(grid, grid_lo, p, NN + p, acc[i].re, acc[i].im, atomic, fresh);
+                else
+                    flush_px(grid, grid_lo, 2 * p, 2 * p + 1, acc[i].re, acc[i].im, atomic, fresh);
             }
             __syncthreads();  // s_item reuse
-            continue;
-        }
-        const int x = Xw + lx;
-#pragma unroll
-        for (int i = 0; i < NG; i++) {
-            const int y = Yw + ly + 8 * i;
-            if (x >= g.N || y >= g.N) continue;
-            const int64_t p = (int64_t)y * g.N + x;
-            if (PLANAR)
-                flush_px(grid, grid_lo, p, NN + p, acc[i].re, acc[i].im, atomic, fresh);
+        };
+        if constexpr (SC == kScBoth) {
+            if (g.S == 64)
+                body(std::integral_constant<int, 64>{});
             else
-                flush_px(grid, grid_lo, 2 * p, 2 * p + 1, acc[i].re, acc[i].im, atomic, fresh);
+                body(std::integral_constant<int, 32>{});
+        } else {
+            body(std::integral_constant<int, SC>{});
         }
-        __syncthreads();  // s_item reuse
     }
 }
 
@@ -1623,6 +1638,7 @@ int run_shape(int T, const OwnerLaunch& L, hipStream_t s)
         // every job at S = 64 or at S = 32: the support as a constant
         if (L.sc == 64) return run_owner_col<G, TA, R, TI, TO, PLANAR, GM, 64>(L, s);
         if (L.sc == 32) return run_owner_col<G, TA, R, TI, TO, PLANAR, GM, 32>(L, s);
+        if (L.sc == kScBoth) return run_owner_col<G, TA, R, TI, TO, PLANAR, GM, kScBoth>(L, s);
         return run_owner_col<G, TA, R, TI, TO, PLANAR, GM>(L, s);
     }
     if (T == 32) return run_owner<2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);
@@ -1896,10 +1912,19 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.smax_owner = smax_owner;
     p.s_common = smin_owner == smax_owner ? smax_owner : 0;
     p.s_common_t = smin_t == smax_t ? smax_t : 0;
-    // Column-kernel jobs of support 32 beside jobs of other supports get a
-    // queue and a launch of their own, so both sets run the instantiations
-    // with the support a constant (the general one, deciding per item, was
-    // 35-40% slower on an S = 64 job).
+    // Otherwise (analytic or constant-tap jobs of other supports) column-
+    // kernel jobs of support 32 get a queue and a launch of their own, so
+    // they keep the S = 32 instantiation (the general one was 35-40% slower
+    // on an S = 64 job).
+    // Column-kernel jobs all at 64 or 32 (every table job, its support
+    // embedded): one launch of the instantiation that holds both constant
+    // forms and picks one per item.
+    if (p.owner && p.s_common == 0 && p.key.T == 64) {
+        bool only = true;
+        for (const TiledJob& t : p.jobs)
+            if (!t.taps && t.g.S != 32 && t.g.S != 64) only = false;
+        if (only) p.s_common = kScBoth;
+    }
     if (p.owner && p.s_common == 0 && p.key.T == 64) {
         int hi = 0, lo = 1 << 30;
         bool has32 = false;

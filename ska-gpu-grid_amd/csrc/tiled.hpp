@@ -191,7 +191,7 @@ struct TiledPlan {
     // own item queue; the other jobs to the owner kernel
     bool taps = false, owner = true;  // some job uses k_taps / the owner kernel
     int smax_owner = 0;               // the largest support among the owner kernel's jobs
-    int s_common = 0;                 // their common support, 0 if they differ
+    int s_common = 0;                 // their common support, 0 if they differ (-1: all 32 or 64)
     int s_common_t = 0;               // the same for the tap kernel's jobs
     size_t lds_taps = 0;
     size_t off_items_t = 0, off_qs_t = 0;
@@ -211,7 +211,8 @@ struct OwnerLaunch {
     int stage_min;  // staging threshold (0: default)
     int wg_per_cu;  // persistent workgroups per CU (0: occupancy limit)
     size_t lds;
-    int sc = 0;  // every job's support when they share one (owner: 32, 64; taps: 8, 16)
+    int sc = 0;  // every job's support when they share one (owner: 32, 64, or -1 for
+                 // 32 and 64 chosen per item; taps: 8, 16)
 };
 
 int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p);

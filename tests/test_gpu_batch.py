@@ -222,3 +222,35 @@ def test_batch_reproducible_mixed_supports(strategy):
         got = g0[0] + 1j * g0[1] if strategy == "window" else g0[..., 0] + 1j * g0[..., 1]
         assert O.rel_errors(got, r)[0] < 1e-12, (strategy, j["S"])
     b.close()
+
+
+@pytest.mark.parametrize("strategy", ["tiled", "gather"])
+def test_batch_coverage_mixed_supports(strategy):
+    """Coverage counts (constant taps, so no support embedding) for supports
+    24, 32 and 64 in one batch: the support-32 jobs run in a second owner
+    launch with their own queue, the others in the general instantiation;
+    counts bit-exact against the oracle, GATHER's split tiles summed through
+    one ordered reduction across both launches."""
+    t = torch()
+    print(f"[batch coverage mixed] {strategy}", flush=True)
+    shapes = [(120, 24, 3000), (150, 32, 4000), (180, 64, 4000), (100, 32, 2000)]
+    jobs, refs = [], []
+    for i, (N, S, n) in enumerate(shapes):
+        uv, vals = O.gen_clustered(n, N, S, mean_r=N / 8, seed=140 + i)
+        gcf = O.config_gcf(S)
+        refs.append(O.grid_zz(uv, vals, N, gcf, S, debug1=True))
+        jobs.append(dict(grid=t.empty((N, N, 2), dtype=t.float64, device="cuda"), uv=dev(uv),
+                         vals=dev(vals), gcf=dev(gcf), N=N, S=S))
+    ctx = migrid.Context()
+    b = migrid.Batch(ctx, jobs, strategy=strategy, debug_coverage=True, vis_per_wave=128,
+                     layout=migrid.LAYOUT_INTERLEAVED)
+    for j in jobs:
+        j["grid"].fill_(float("nan"))
+    t.cuda.synchronize()
+    b.run()
+    t.cuda.synchronize()
+    for j, r in zip(jobs, refs):
+        g = j["grid"].cpu().numpy()
+        np.testing.assert_array_equal(g[..., 0], r[0])
+        np.testing.assert_array_equal(g[..., 1], r[1])
+    b.close()
