@@ -92,9 +92,10 @@ typedef struct migrid_opts {
     int32_t vis_per_wave;   /* TILED work-item size, 0 = auto */
     double gcf_sigma;       /* ANALYTIC: Gaussian sigma in pixels (0: S/6) */
     double gcf_beta;        /* ANALYTIC: quadratic phase coefficient */
-    int32_t stage_min;      /* TILED tuning: a sub-kernel with fewer visibilities
-                               for a tile is read from HBM instead of staged in
-                               LDS (0 = auto) */
+    int32_t stage_min;      /* TILED tuning, square tiles (tile 16 or 32): a
+                               sub-kernel with fewer visibilities for a tile is
+                               read from HBM instead of staged in LDS (0 = auto);
+                               the 128 x 64 tiles stage every sub-kernel */
     int32_t wg_per_cu;      /* TILED tuning: persistent workgroups per CU (0 = auto) */
     int32_t tap_smax;       /* TILED: supports up to this (at most 16) are gridded
                                one tap per lane into an LDS tile; 0 = 16, -1 = never */

@@ -454,7 +454,6 @@ __host__ __device__ constexpr int stage_pitch(int S)
     return stage_rows(S) + ((8 - stage_rows(S) % 32) + 32) % 32;
 }
 constexpr int kStageMinDefault = 4;  // fewer visibilities for a k: read the table from HBM
-constexpr int kStageMinMixed = 2;    // the same for supports 32, 64 in a launch of mixed supports
 // LDS of a workgroup beside the kernel's static arrays (160 KB per CU)
 constexpr size_t kMaxLds = 160 * 1024 - 4096;
 
@@ -922,7 +921,7 @@ template <typename G, typename TA, typename R, typename TI, typename TO, bool PL
 __global__ __launch_bounds__(1024)
 __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner_col(
     const JobDesc* __restrict__ jobs, const Item* __restrict__ items,
-    QueueState* __restrict__ qs, int smax, int stage_min)
+    QueueState* __restrict__ qs, int smax)
 {
     using P = typename Stage<G>::P;  // product type
     using V = decltype(+((const R*)nullptr)->vr);  // record value type
@@ -1010,15 +1009,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner_col(
                                     ? jobs[job].tcount[(int64_t)tile * kSubKernels + lane]
                                     : 0;
                 nz_mask = __ballot(cnt > 0);
-                // (with the support a constant every non-empty sub-kernel is
-                // staged; in a launch of mixed supports, items of supports 32 and
-                // 64 read single visibilities' taps from memory by default, the
-                // autotuner's choice for the 64-job batch)
-                const bool all = GM != 0 || SX != 0;
-                const int smin = stage_min > 0 ? stage_min
-                                 : (S == 32 || S == 64) ? kStageMinMixed
-                                                        : kStageMinDefault;
-                st_mask = all ? nz_mask : __ballot(cnt >= smin && cnt > 0);
+                // every non-empty sub-kernel is staged (table jobs always run
+                // with the support a constant, whose registers a direct-from-
+                // memory path would take)
+                st_mask = nz_mask;
             }
             // sub-kernel rows this wave stages: w, w + NW, ... (one column per
             // lane); the next staged sub-kernel is fetched into registers while
@@ -1178,34 +1172,6 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner_col(
                                     Stage<G>::get(b[8 * i * pitch], gr, gi);
                                     mac1<P, TA>(acc[i], vr, vi, gr, gi);
                                 }
-                            }
-                        }
-                    }
-                } else if constexpr (SX == 0) {
-                    // a sparse sub-kernel read from the table in memory (any
-                    // support; with the support a constant every sub-kernel is
-                    // staged, and the registers this path would hold go to the
-                    // staged loop)
-                    GlobalPtr<G> gk = (GlobalPtr<G>)(uintptr_t)(gtab + (int64_t)k * S * S);
-                    for (int t = 0; t < nry; t++) {
-                        const int A = max(0, t - D), B = min(NG - 1, t);
-                        const int r0 = offs[key0 + t * nb], r1 = offs[key0 + t * nb + ncx];
-                        for (int r = r0; r < r1; r++) {
-                            const uint32_t q = rec[r].q;
-                            const int c = S - 1 - ((int)(q & 0xffffu) - Xw) + lx;
-                            const int rr0 = S - 1 - ((int)(q >> 16) - Yw) + ly;
-                            const P vr = (P)rec[r].vr, vi = (P)rec[r].vi;
-                            // groups in fours, 32-bit element offsets: few taps
-                            // and addresses live at a time
-    #pragma unroll
-                            for (int i = 0; i < NG; i++) {
-                                if (i < A || i > B) continue;
-                                const int rr = rr0 + 8 * i;
-                                P gr = (P)0, gi = (P)0;
-                                if ((unsigned)c < (unsigned)S && (unsigned)rr < (unsigned)S)
-                                    Stage<G>::get(gload<G>(gk, rr * S + c), gr, gi);
-                                mac1<P, TA>(acc[i], vr, vi, gr, gi);
-                                if (i % 4 == 3) __builtin_amdgcn_sched_barrier(0);
                             }
                         }
                     }
@@ -1623,7 +1589,7 @@ int run_owner_col(const OwnerLaunch& L, hipStream_t s)
     }
     if (L.wg_per_cu > 0) per_cu = L.wg_per_cu;
     hipLaunchKernelGGL(kern, dim3(ncu * per_cu), dim3(1024), L.lds, s, L.jobs, L.items, L.qs,
-                       L.smax, L.stage_min);
+                       L.smax);
     MG_HIP(hipGetLastError());
     return MIGRID_OK;
 }
@@ -1639,7 +1605,12 @@ int run_shape(int T, const OwnerLaunch& L, hipStream_t s)
         if (L.sc == 64) return run_owner_col<G, TA, R, TI, TO, PLANAR, GM, 64>(L, s);
         if (L.sc == 32) return run_owner_col<G, TA, R, TI, TO, PLANAR, GM, 32>(L, s);
         if (L.sc == kScBoth) return run_owner_col<G, TA, R, TI, TO, PLANAR, GM, kScBoth>(L, s);
-        return run_owner_col<G, TA, R, TI, TO, PLANAR, GM>(L, s);
+        // table jobs always run embedded in 32 or 64 (tiled_plan); only the
+        // computed forms keep other supports
+        if constexpr (GM == 0)
+            return fail(MIGRID_ENOTSUP, "tiled: internal: table jobs of mixed supports");
+        else
+            return run_owner_col<G, TA, R, TI, TO, PLANAR, GM>(L, s);
     }
     if (T == 32) return run_owner<2, 2, G, TA, R, TI, TO, PLANAR, GM>(L, s);
     if (T == 16) return run_owner<1, 1, G, TA, R, TI, TO, PLANAR, GM>(L, s);

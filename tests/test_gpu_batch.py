@@ -120,8 +120,10 @@ def test_staged_and_direct_paths(stage_min, arith, tol):
     grid = t.empty((N, N, 2), dtype=t.float64, device="cuda")
     ctx = migrid.Context()
     for wg in (0, 1):
+        # square tiles (tile option 32): the kernel with both coefficient
+        # paths; the 128 x 64 tiles stage every sub-kernel
         ctx.grid(grid, dev(uv), dev(vals), dev(gcf), N, S, strategy="tiled", arith=arith,
-                 stage_min=stage_min, wg_per_cu=wg)
+                 stage_min=stage_min, wg_per_cu=wg, tile=32)
         t.cuda.synchronize()
         assert O.rel_errors(cplx(grid), ref)[0] < tol
 
