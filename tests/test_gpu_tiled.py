@@ -219,8 +219,8 @@ def test_gather_and_window_are_bitwise_reproducible(strategy):
     assert O.rel_errors(got - (base[0] + 1j * base[1]), ref)[0] < 1e-12
 
 
-@pytest.mark.parametrize("S,N,strategy", [(6, 90, "gather"), (24, 150, "tiled"), (34, 120, "gather"),
-                                          (40, 160, "window"), (62, 200, "tiled")])
+@pytest.mark.parametrize("S,N,strategy", [(2, 50, "gather"), (6, 90, "gather"), (24, 150, "tiled"),
+                                          (34, 120, "gather"), (40, 160, "window"), (62, 200, "tiled")])
 def test_embedded_supports(S, N, strategy):
     """Supports other than 32 and 64 run on the column kernel embedded in the
     next of them (the table surrounded by zero taps, every tap on its own
