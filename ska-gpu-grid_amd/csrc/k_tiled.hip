@@ -2276,15 +2276,40 @@ static int clear_only(const TiledPlan& p, const GridArgs& a, hipStream_t s)
     return MIGRID_OK;
 }
 
+// Visibilities one plan of this job's form takes: its packed records stay
+// under 2^32 bytes (32-byte records with FP64 tiles, 16-byte otherwise).
+int64_t tiled_chunk_limit(const GridArgs& a, const migrid_opts& o)
+{
+    const bool f32arith = o.arith == MIGRID_ARITH_F32 || o.arith == MIGRID_ARITH_F32_BF16GCF;
+    const bool rec64 = o.arith != MIGRID_ARITH_F32_BF16GCF && !f32arith && a.in_type != MIGRID_F32;
+    return (((int64_t)1 << 32) - 1) / (int64_t)(rec64 ? sizeof(RecF64) : sizeof(RecF32));
+}
+
 int launch_tiled(migrid_ctx* ctx, const GridArgs& a, const migrid_opts& o, hipStream_t s)
 {
     if (!ctx) return fail(MIGRID_EINVAL, "tiled strategies need a context");
-    TiledPlan p;
-    MG_TRY(tiled_plan(&a, 1, o, p));
-    if (a.npts == 0) return clear_only(p, a, s);
-    MG_TRY(ctx->bins.ensure(p.bin_bytes));
-    MG_TRY(ctx->records.ensure(p.arena_bytes));
-    return run_single(p, a, (char*)ctx->records.p, ctx->bins.p, &ctx->prof, s);
+    // a job past one plan's records runs as consecutive chunks of its
+    // visibilities, the first into the grid as the caller asked (fresh or
+    // accumulating), the rest added to it; the chunk order is fixed, so the
+    // ordered-reduction strategies stay reproducible
+    const int64_t lim = tiled_chunk_limit(a, o);
+    const size_t esz = a.in_type == MIGRID_F64 ? 8 : 4;
+    for (int64_t done = 0;;) {
+        GridArgs c = a;
+        migrid_opts oc = o;
+        c.npts = std::min<int64_t>(a.npts - done, lim);
+        c.uv = (const char*)a.uv + done * 2 * esz;
+        c.vals = (const char*)a.vals + done * 2 * esz;
+        if (done > 0) oc.accumulate = 1;
+        TiledPlan p;
+        MG_TRY(tiled_plan(&c, 1, oc, p));
+        if (c.npts == 0) return clear_only(p, c, s);
+        MG_TRY(ctx->bins.ensure(p.bin_bytes));
+        MG_TRY(ctx->records.ensure(p.arena_bytes));
+        MG_TRY(run_single(p, c, (char*)ctx->records.p, ctx->bins.p, &ctx->prof, s));
+        done += c.npts;
+        if (done >= a.npts) return MIGRID_OK;
+    }
 }
 
 int launch_tiled_scratch(const GridArgs& a, const migrid_opts& o, void* scratch, size_t cap,

@@ -244,17 +244,32 @@ def test_embedded_supports(S, N, strategy):
         assert O.rel_errors(g, ref)[0] < tol, (arith, S)
 
 
-def test_record_bytes_limit():
-    """A tiled plan whose packed records would pass 2^32 bytes (the column
-    kernel's byte offsets are 32-bit) is refused before any work: 2^27
-    visibilities of 32-byte FP64 records."""
+def test_job_past_one_plan_is_chunked():
+    """A job whose packed records would pass 2^32 bytes (the column kernel's
+    byte offsets are 32-bit) runs as consecutive chunks of at most
+    2^32 / 32 - 1 visibilities (FP64 records), the later ones added to the
+    grid: with GATHER the result is bit for bit the one two calls give, the
+    first on the leading chunk into a fresh grid, the second on the rest
+    accumulating."""
     t = torch()
-    n = 1 << 27
-    uv = t.empty((n, 2), dtype=t.float64, device="cuda")
-    vals = t.empty((n, 2), dtype=t.float64, device="cuda")
-    grid = t.empty((64, 64, 2), dtype=t.float64, device="cuda")
-    gcf = dev(O.init_gcf(8))
-    with pytest.raises(migrid.MigridError) as e:
-        migrid.Context().grid(grid, uv, vals, gcf, 64, 8, strategy="gather")
-    assert "2^32 bytes" in str(e.value)
+    lim = ((1 << 32) - 1) // 32
+    n = lim + 1000
+    N, S = 256, 8
+    print(f"[chunked job] n={n}", flush=True)
+    g = t.Generator(device="cuda")
+    g.manual_seed(3)
+    uv = t.rand((n, 2), dtype=t.float64, device="cuda", generator=g) * (N - 2 * S) + S
+    vals = t.randn((n, 2), dtype=t.float64, device="cuda", generator=g)
+    gcf = dev(O.init_gcf(S))
+    ctx = migrid.Context()
+    whole = t.full((N, N, 2), float("nan"), dtype=t.float64, device="cuda")
+    ctx.grid(whole, uv, vals, gcf, N, S, strategy="gather", layout=migrid.LAYOUT_INTERLEAVED)
+    parts = t.full((N, N, 2), float("nan"), dtype=t.float64, device="cuda")
+    ctx.grid(parts, uv[:lim], vals[:lim], gcf, N, S, strategy="gather",
+             layout=migrid.LAYOUT_INTERLEAVED)
+    ctx.grid(parts, uv[lim:], vals[lim:], gcf, N, S, strategy="gather",
+             layout=migrid.LAYOUT_INTERLEAVED, accumulate=True)
+    t.cuda.synchronize()
+    assert t.isfinite(whole).all()
+    assert t.equal(whole, parts)
     del uv, vals
