@@ -2278,7 +2278,7 @@ static int clear_only(const TiledPlan& p, const GridArgs& a, hipStream_t s)
 
 // Visibilities one plan of this job's form takes: its packed records stay
 // under 2^32 bytes (32-byte records with FP64 tiles, 16-byte otherwise).
-int64_t tiled_chunk_limit(const GridArgs& a, const migrid_opts& o)
+static int64_t tiled_chunk_limit(const GridArgs& a, const migrid_opts& o)
 {
     const bool f32arith = o.arith == MIGRID_ARITH_F32 || o.arith == MIGRID_ARITH_F32_BF16GCF;
     const bool rec64 = o.arith != MIGRID_ARITH_F32_BF16GCF && !f32arith && a.in_type != MIGRID_F32;
