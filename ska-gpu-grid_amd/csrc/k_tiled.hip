@@ -921,6 +921,7 @@ __device__ __forceinline__ void static_for(F&& f)
 // makes the bin rows' group ranges and the LDS offsets constants; kScBoth:
 // every job at 64 or 32, the constant chosen per item.
 constexpr int kScBoth = -1;
+
 template <typename G, typename TA, typename R, typename TI, typename TO, bool PLANAR, int GM,
           int SC>
 __global__ __launch_bounds__(1024)
@@ -941,7 +942,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner_col(
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6) & (NW - 1);
     const int lx = lane & 7, ly = lane >> 3;
 
-    for (int e = tid; e < col_rows(smax) * col_pitch(smax); e += NT)
+    // (two support-32 layouts side by side when S = 32 sub-kernels are staged in pairs)
+    const int lds_elems = max(col_rows(smax) * col_pitch(smax),
+                              smax >= 32 ? 2 * col_rows(32) * col_pitch(32) : 0);
+    for (int e = tid; e < lds_elems; e += NT)
         sg[e] = Stage<G>::make(0.0, 0.0);
     int staged_s = -1;  // support of the layout now in LDS
 
@@ -979,8 +983,13 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner_col(
                 __syncthreads();
                 if (staged_s > 0) {
                     const int ol = col_lead(staged_s), op = col_pitch(staged_s);
-                    for (int e = tid; e < staged_s * staged_s; e += NT)
-                        sg[(e / staged_s + ol) * op + e % staged_s + ol] = Stage<G>::make(0.0, 0.0);
+                    // (the second of a pair's layouts too, at S = 32)
+                    const int nl = GM == 0 && staged_s == 32 ? 2 : 1;
+                    for (int e = tid; e < nl * staged_s * staged_s; e += NT) {
+                        const int l = e / (staged_s * staged_s), f = e % (staged_s * staged_s);
+                        sg[l * col_rows(32) * col_pitch(32) + (f / staged_s + ol) * op + f % staged_s + ol] =
+                            Stage<G>::make(0.0, 0.0);
+                    }
                     __syncthreads();
                 }
                 if (GM == 1)
@@ -1033,7 +1042,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner_col(
                     if (r < S && lane < S) pf[p] = gload<G>(src, r * S + lane);
                 }
             };
-            if (PF && st_mask) fetch_rows(__builtin_ctzll(st_mask));
+            // S = 32 (table): two sub-kernels staged side by side per barrier
+            // pair (layouts at 0 and col_rows(32) col_pitch(32) elements); the
+            // four prefetch registers hold both, two rows of each per wave
+            constexpr bool PAIR = SX == 32 && GM == 0;
+            if (PF && !PAIR && st_mask) fetch_rows(__builtin_ctzll(st_mask));
 
             // Per lane: the staged element of group 0 for q = 0, in bytes; a
             // visibility shifts it back by its record offset (q_y P + q_x
@@ -1055,6 +1068,121 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner_col(
             };
             int bnd_next = nz_mask ? load_bounds(__builtin_ctzll(nz_mask)) : 0;
 
+            // the rows of one sub-kernel, staged at lbx, bounds bndx, with the
+            // support a constant SX: each bin row's groups and the group
+            // offsets are constants
+            auto rows_const = [&](auto sx, const char* lbx, int bndx) {
+                constexpr int SX = decltype(sx)::value;
+                constexpr int NR = col_nrows(SX), DC = col_depth(SX), PC = col_pitch(SX);
+                constexpr uint32_t RS = sizeof(R);
+                ConstPtr<char> rbase = (ConstPtr<char>)rec;
+                // the bounds as byte offsets into the records, and a mask
+                // of the non-empty rows (bit 2t: row t), so an empty row
+                // costs one bit test
+                const uint32_t bb = (uint32_t)bndx * RS;
+                const uint32_t bb_next =
+                    (uint32_t)__builtin_amdgcn_mov_dpp((int)bb, 0xB1, 0xF, 0xF, false);
+                const uint64_t rows_on = __ballot(!(lane & 1) && bb < bb_next);
+                static_for<NR>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr int A = t - DC > 0 ? t - DC : 0;
+                    constexpr int B = t < NG - 1 ? t : NG - 1;
+                    if (!((rows_on >> (2 * t)) & 1)) return;
+                    uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)bb, 2 * t);
+                    const uint32_t oe = (uint32_t)__builtin_amdgcn_readlane((int)bb, 2 * t + 1);
+                    // the taps of one visibility
+                    auto one = [&](uint32_t off, V vr, V vi) {
+                        const G* b = (const G*)(lbx - off);
+                        P gr[NG], gi[NG];
+#pragma unroll
+                        for (int i = A; i <= B; i++) Stage<G>::get(b[8 * i * PC], gr[i], gi[i]);
+#pragma unroll
+                        for (int i = A; i <= B; i++)
+                            mac1<P, TA>(acc[i], (P)vr, (P)vi, gr[i], gi[i]);
+                    };
+                    // pairs of visibilities from one scalar record load,
+                    // then the row's odd last one alone
+                    const uint32_t oe1 = oe - RS;
+                    for (; o < oe1; o += 2 * RS) {
+                        uint32_t off[2];
+                        V vr[2], vi[2];
+                        pair_at<R>(rbase, o, off, vr, vi);
+                        one(off[0], vr[0], vi[0]);
+                        // rows of many groups: the second visibility's
+                        // taps are read after the first one's products,
+                        // so one visibility's taps are live at a time
+                        if constexpr (B - A + 1 > 4) __builtin_amdgcn_sched_barrier(0);
+                        one(off[1], vr[1], vi[1]);
+                    }
+                    if (o == oe1) {
+                        uint32_t off;
+                        V vr, vi;
+                        rec_at<R>(rbase, o, off, vr, vi);
+                        one(off, vr, vi);
+                    }
+                });
+            };
+
+            if constexpr (PAIR) {
+                constexpr int kPairOff = col_rows(32) * col_pitch(32);
+                constexpr int PRS = PR / 2;
+                auto fetch2 = [&](int kk, int slot) {
+                    GlobalPtr<G> src = (GlobalPtr<G>)(uintptr_t)(gtab + (int64_t)kk * 32 * 32);
+    #pragma unroll
+                    for (int p = 0; p < PRS; p++) {
+                        const int r = w + p * NW;
+                        if (lane < 32) pf[slot * PRS + p] = gload<G>(src, r * 32 + lane);
+                    }
+                };
+                const char* lb2 = lb + (size_t)kPairOff * sizeof(G);
+                uint64_t todo = nz_mask;
+                int b1 = 0, b2 = 0;
+                if (todo) {
+                    uint64_t t = todo;
+                    const int k1 = __builtin_ctzll(t);
+                    fetch2(k1, 0);
+                    b1 = load_bounds(k1);
+                    t &= t - 1;
+                    if (t) {
+                        fetch2(__builtin_ctzll(t), 1);
+                        b2 = load_bounds(__builtin_ctzll(t));
+                    }
+                }
+                while (todo) {
+                    todo &= todo - 1;
+                    const bool has2 = todo != 0;
+                    if (has2) todo &= todo - 1;
+                    const int bnd1 = b1, bnd2 = b2;
+                    {
+                        uint64_t t = todo;
+                        if (t) {
+                            b1 = load_bounds(__builtin_ctzll(t));
+                            t &= t - 1;
+                            if (t) b2 = load_bounds(__builtin_ctzll(t));
+                        }
+                    }
+                    __syncthreads();  // the previous pair no longer read
+    #pragma unroll
+                    for (int p = 0; p < PRS; p++) {
+                        const int r = w + p * NW;
+                        if (lane < 32) {
+                            sg[(r + L) * pitch + lane + L] = pf[p];
+                            if (has2) sg[kPairOff + (r + L) * pitch + lane + L] = pf[PRS + p];
+                        }
+                    }
+                    {
+                        uint64_t t = todo;
+                        if (t) {
+                            fetch2(__builtin_ctzll(t), 0);
+                            t &= t - 1;
+                            if (t) fetch2(__builtin_ctzll(t), 1);
+                        }
+                    }
+                    __syncthreads();  // both staged
+                    rows_const(std::integral_constant<int, 32>{}, lb, bnd1);
+                    if (has2) rows_const(std::integral_constant<int, 32>{}, lb2, bnd2);
+                }
+            } else {
             for (uint64_t todo = nz_mask; todo; todo &= todo - 1) {
                 const int k = __builtin_ctzll(todo);
                 const bool staged = (st_mask >> k) & 1;
@@ -1099,68 +1227,15 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner_col(
                     }
                     __syncthreads();
                 }
-                // the support a constant SX: each bin row's groups and the group
-                // offsets are constants
-                auto rows_const = [&](auto sx) {
-                    constexpr int SX = decltype(sx)::value;
-                    constexpr int NR = col_nrows(SX), DC = col_depth(SX), PC = col_pitch(SX);
-                    constexpr uint32_t RS = sizeof(R);
-                    ConstPtr<char> rbase = (ConstPtr<char>)rec;
-                    // the bounds as byte offsets into the records, and a mask
-                    // of the non-empty rows (bit 2t: row t), so an empty row
-                    // costs one bit test
-                    const uint32_t bb = (uint32_t)bnd * RS;
-                    const uint32_t bb_next =
-                        (uint32_t)__builtin_amdgcn_mov_dpp((int)bb, 0xB1, 0xF, 0xF, false);
-                    const uint64_t rows_on = __ballot(!(lane & 1) && bb < bb_next);
-                    static_for<NR>([&](auto tc) {
-                        constexpr int t = decltype(tc)::value;
-                        constexpr int A = t - DC > 0 ? t - DC : 0;
-                        constexpr int B = t < NG - 1 ? t : NG - 1;
-                        if (!((rows_on >> (2 * t)) & 1)) return;
-                        uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)bb, 2 * t);
-                        const uint32_t oe = (uint32_t)__builtin_amdgcn_readlane((int)bb, 2 * t + 1);
-                        // the taps of one visibility
-                        auto one = [&](uint32_t off, V vr, V vi) {
-                            const G* b = (const G*)(lb - off);
-                            P gr[NG], gi[NG];
-    #pragma unroll
-                            for (int i = A; i <= B; i++) Stage<G>::get(b[8 * i * PC], gr[i], gi[i]);
-    #pragma unroll
-                            for (int i = A; i <= B; i++)
-                                mac1<P, TA>(acc[i], (P)vr, (P)vi, gr[i], gi[i]);
-                        };
-                        // pairs of visibilities from one scalar record load,
-                        // then the row's odd last one alone
-                        const uint32_t oe1 = oe - RS;
-                        for (; o < oe1; o += 2 * RS) {
-                            uint32_t off[2];
-                            V vr[2], vi[2];
-                            pair_at<R>(rbase, o, off, vr, vi);
-                            one(off[0], vr[0], vi[0]);
-                            // rows of many groups: the second visibility's
-                            // taps are read after the first one's products,
-                            // so one visibility's taps are live at a time
-                            if constexpr (B - A + 1 > 4) __builtin_amdgcn_sched_barrier(0);
-                            one(off[1], vr[1], vi[1]);
-                        }
-                        if (o == oe1) {
-                            uint32_t off;
-                            V vr, vi;
-                            rec_at<R>(rbase, o, off, vr, vi);
-                            one(off, vr, vi);
-                        }
-                    });
-                };
                 if (staged) {
                     if constexpr (SX != 0) {
-                        rows_const(std::integral_constant<int, SX>{});
+                        rows_const(std::integral_constant<int, SX>{}, lb, bnd);
                     } else if (S == 64) {
                         // jobs of mixed supports: the common ones take the
                         // constant-support rows, decided once per item
-                        rows_const(std::integral_constant<int, 64>{});
+                        rows_const(std::integral_constant<int, 64>{}, lb, bnd);
                     } else if (S == 32) {
-                        rows_const(std::integral_constant<int, 32>{});
+                        rows_const(std::integral_constant<int, 32>{}, lb, bnd);
                     } else {
                         // any support: the groups of bin row t are tested per
                         // visibility (wave-uniform branches)
@@ -1182,6 +1257,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(G) == 16 ? 4 : 8))) void k_owner_col(
                     }
                 }
             }
+            }  // PAIR
             // flush: a split tile of the ordered-reduction mode writes its
             // partial tile (128 x 64 accumulators) to its slot
             if (pslot >= 0) {
@@ -2015,9 +2091,12 @@ int tiled_plan(const GridArgs* a, int n, const migrid_opts& o, TiledPlan& p)
     p.stage_min = o.stage_min;
     p.wg_per_cu = o.wg_per_cu;
     const size_t gsz = p.key.stage == kStageBf16 ? 4 : p.key.stage == kStageF32 ? 8 : 16;
+    // (the column kernel stages S = 32 sub-kernels in pairs: two layouts)
     p.lds = !p.owner ? 0
-            : p.key.T == 64 ? (size_t)col_rows(smax_owner) * col_pitch(smax_owner) * gsz
-                            : (size_t)stage_rows(smax_owner) * stage_pitch(smax_owner) * gsz;
+            : p.key.T == 64
+                ? (size_t)std::max(col_rows(smax_owner) * col_pitch(smax_owner),
+                                   smax_owner >= 32 ? 2 * col_rows(32) * col_pitch(32) : 0) * gsz
+                : (size_t)stage_rows(smax_owner) * stage_pitch(smax_owner) * gsz;
     p.lds_taps = p.taps ? (size_t)2 * p.key.T * tap_pitch(p.key.T) * sizeof(double) : 0;
     return MIGRID_OK;
 }
