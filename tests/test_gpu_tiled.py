@@ -244,6 +244,44 @@ def test_embedded_supports(S, N, strategy):
         assert O.rel_errors(g, ref)[0] < tol, (arith, S)
 
 
+@pytest.mark.parametrize("strategy", STRATS)
+@pytest.mark.parametrize("S", [32, 24])
+def test_sparse_support32_pairs(S, strategy):
+    """At S = 32 (and supports embedded in it) the column kernel stages two
+    sub-kernels per barrier pair.  A few visibilities in one corner of the
+    grid give tiles with one, two or an odd number of non-empty sub-kernels
+    (the last pair half empty), items cut down to single sub-kernels, and
+    one run where every visibility has the same sub-kernel.  FP64 within
+    1e-12 of the oracle, FP32 tiles within 1e-5; the grid starts as NaN."""
+    t = torch()
+    N = 192
+    gcf = O.init_gcf(S)
+    planar = strategy == "window"
+    ctx = migrid.Context()
+    rng = np.random.default_rng(S * 3 + len(strategy))
+    cases = []
+    for n in (1, 2, 3, 5, 9, 33):
+        uv = rng.uniform(40.0, 52.0, size=(n, 2))
+        vals = rng.standard_normal((n, 2))
+        cases.append((f"n={n}", uv, vals))
+    uv = np.floor(rng.uniform(40.0, 140.0, size=(7, 2))) + 0.3125
+    cases.append(("one sub-kernel", uv, rng.standard_normal((7, 2))))
+    for name, uv, vals in cases:
+        print(f"[support-32 pairs] S={S} {strategy} {name}", flush=True)
+        ref = cplx(O.grid_zz(uv, vals, N, gcf, S))
+        runs = [(migrid.ARITH_NATIVE, 1e-12, {})]
+        if strategy == "tiled":
+            runs += [(migrid.ARITH_NATIVE, 1e-12, {"vis_per_wave": 1}), (migrid.ARITH_F32, 1e-5, {})]
+        for arith, tol, kw in runs:
+            grid = t.full((2, N, N) if planar else (N, N, 2), float("nan"), dtype=t.float64,
+                          device="cuda")
+            ctx.grid(grid, dev(uv), dev(vals), dev(gcf), N, S, strategy=strategy, arith=arith, **kw)
+            t.cuda.synchronize()
+            g = grid.cpu().numpy()
+            g = g[0] + 1j * g[1] if planar else g[..., 0] + 1j * g[..., 1]
+            assert O.rel_errors(g, ref)[0] < tol, (name, arith, kw)
+
+
 def test_job_past_one_plan_is_chunked():
     """A job whose packed records would pass 2^32 bytes (the column kernel's
     byte offsets are 32-bit) runs as consecutive chunks of at most
